@@ -1,0 +1,80 @@
+"""Build the gfx950 HIP library libgpehip.so in-tree.
+
+    python gpe-matlab_amd/build.py [-j JOBS]
+
+Every pencil length is compiled as its own object (sweep_kernels.hip with
+-DGPE_N=<n>) so the build parallelises; the objects are linked with hipcc into
+gpe-matlab_amd/gpe_hip/libgpehip.so, which the ctypes layer loads.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "gpe_hip", "libgpehip.so")
+LENGTHS = (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)
+SOURCES = ("state_kernels.hip", "gpe_capi.hip", "spinor.hip", "adjoint.hip", "batch.hip")
+ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc():
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found (ROCm toolchain required to build the HIP library)")
+    return p
+
+
+def flags():
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-command-line-argument",
+            # packed FP32 instructions are disabled for all device code
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", os.path.join(HERE, "..", "include"), "-I", CSRC]
+
+
+def compile_one(src, obj, extra=()):
+    cmd = [hipcc(), *flags(), *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    tasks = []
+    for n in LENGTHS:
+        tasks.append(("sweep_kernels.hip", os.path.join(BUILD, f"sweep_{n}.o"), (f"-DGPE_N={n}",)))
+    for s in SOURCES:
+        if os.path.exists(os.path.join(CSRC, s)):
+            tasks.append((s, os.path.join(BUILD, s.replace(".hip", ".o")), ()))
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(compile_one, *t) for t in tasks]
+        for f in futs:
+            objs.append(f.result())
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT} ({len(objs)} objects)")
+    return OUT
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", "--jobs", type=int, default=None)
+    a = ap.parse_args()
+    try:
+        build(a.jobs)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

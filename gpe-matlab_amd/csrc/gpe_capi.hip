@@ -1,0 +1,403 @@
+// C-ABI implementation: device state, tables and the split-step drivers.
+//
+// The drivers issue the fused pencil sweeps on one HIP stream.  Per 3D step:
+//   x: inverse . phase . forward   (S_OPEN / S_MID / S_CLOSE)
+//   y: forward                     (S_FWD)
+//   z: forward . kinetic . inverse (S_KIN)
+//   y: inverse                     (S_INV)
+// and per 2D step:  x: inverse . phase . forward,  y: forward . kinetic . inverse.
+// Block-boundary scalars (norm, renormalisation, chemical potential) are reduced
+// and updated on the device; the host reads them once per outer step.
+#include "handle.h"
+#include "spinor.h"
+
+using namespace gpe_detail;
+
+// =========================================================================== C ABI
+extern "C" {
+
+int gpe_device_info(char* buf, int len) {
+  if (!buf || len <= 0) return GPE_EINVAL;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    snprintf(buf, len, "no HIP device");
+    return GPE_EHIP;
+  }
+  snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+           prop.totalGlobalMem / 1073741824.0);
+  return GPE_OK;
+}
+
+int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
+  if (!d || !out) return GPE_EINVAL;
+  *out = nullptr;
+  if (d->ndims != 2 && d->ndims != 3) return GPE_EINVAL;
+  for (int a = 0; a < d->ndims; ++a) {
+    if (!gpe::sweep_length_supported(d->n[a]) || !(d->xmax[a] > 0)) return GPE_EINVAL;
+  }
+  gpe_handle* h = new gpe_handle();
+  const char* dev = getenv("GPE_DEVICE");
+  h->device = dev ? atoi(dev) : 0;
+  if (hipSetDevice(h->device) != hipSuccess) {
+    delete h;
+    return GPE_EHIP;
+  }
+  h->ndims = d->ndims;
+  h->nx = d->n[0];
+  h->ny = d->n[1];
+  h->nz = d->ndims == 3 ? d->n[2] : 1;
+  h->npts = (long long)h->nx * h->ny * h->nz;
+  const int nn[3] = {h->nx, h->ny, h->nz};
+  double dd[3] = {0, 0, 1};
+  for (int a = 0; a < h->ndims; ++a) {
+    std::vector<double> x = linspace(-d->xmax[a], d->xmax[a], nn[a]);
+    dd[a] = x[1] - x[0];
+    h->ax[a].k = kvec(x, dd[a]);
+    h->ax[a].n = nn[a];
+  }
+  h->dx = dd[0];
+  h->dy = dd[1];
+  h->dz = h->ndims == 3 ? dd[2] : 0;
+  h->w = h->ndims == 3 ? h->dx * h->dy * h->dz : h->dx * h->dy;
+  const long long ny = h->ny, nx = h->nx, nz = h->nz;
+  // x pencils: columns (iy, iz), base = iy + ny*nx*iz, stride ny
+  h->ax[0].stride = ny; h->ax[0].ncols = ny * nz; h->ax[0].A = ny; h->ax[0].sa = 1; h->ax[0].sb = ny * nx;
+  // y pencils: columns (ix, iz), base = col*ny, stride 1
+  h->ax[1].stride = 1; h->ax[1].ncols = nx * nz; h->ax[1].A = 1; h->ax[1].sa = 0; h->ax[1].sb = ny;
+  // z pencils: columns (iy, ix), base = col, stride nx*ny
+  h->ax[2].stride = nx * ny; h->ax[2].ncols = nx * ny; h->ax[2].A = nx * ny; h->ax[2].sa = 1; h->ax[2].sb = 0;
+
+  int maxwg = 2048;
+  for (int a = 0; a < h->ndims; ++a) {
+    const int C = gpe::sweep_cols_per_block(nn[a]);
+    const long long nb = (h->ax[a].ncols + C - 1) / C;
+    if (nb > maxwg) maxwg = (int)nb;
+  }
+  h->maxwg = maxwg;
+
+  auto bail = [&](int code, const char* what) {
+    gpe_destroy(h);
+    (void)what;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(GPE_EHIP, "stream");
+  if (hipMalloc(&h->psi[0], h->npts * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM, "psi");
+  if (hipMalloc(&h->V[0], h->npts * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "V");
+  if (hipMalloc(&h->partials, (size_t)gpe::SL_NSLOT * maxwg * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "partials");
+  if (hipMalloc(&h->st, sizeof(DevState)) != hipSuccess) return bail(GPE_ENOMEM, "state");
+  const int ma = h->ndims == 3 ? 2 : 1;
+  const size_t ncol_mid = (size_t)h->ax[ma].ncols;
+  if (hipMalloc(&h->kcol, ncol_mid * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM, "kcol");
+  if (hipMalloc(&h->kax, h->ax[ma].n * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM, "kax");
+  for (int a = 0; a < h->ndims; ++a) {
+    const int n = nn[a];
+    std::vector<double2> tw = twiddles(n);
+    std::vector<double> ke(n);
+    for (int i = 0; i < n; ++i) ke[i] = h->ax[a].k[i] * h->ax[a].k[i] / 2.0 / (double)n;
+    if (hipMalloc(&h->ax[a].tw, n * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM, "tw");
+    if (hipMalloc(&h->ax[a].keax, n * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "ke");
+    if (hipMemcpy(h->ax[a].tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "tw");
+    if (hipMemcpy(h->ax[a].keax, ke.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "ke");
+  }
+  DevState s{};
+  s.scale = 1.0;
+  s.mu_itp = 1.0;
+  if (hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "state");
+  if (hipMemset(h->psi[0], 0, h->npts * sizeof(double2)) != hipSuccess) return bail(GPE_EHIP, "memset");
+  h->prm.ncomp = 1;
+  h->prm.n_recalc = 10;
+  *out = h;
+  return GPE_OK;
+}
+
+int gpe_destroy(gpe_handle* h) {
+  if (!h) return GPE_OK;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (int c = 0; c < 2; ++c) {
+    hipFree(h->psi[c]);
+    hipFree(h->V[c]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    hipFree(h->ax[a].tw);
+    hipFree(h->ax[a].keax);
+  }
+  hipFree(h->cpl);
+  hipFree(h->kcol);
+  hipFree(h->kax);
+  hipFree(h->kcol2);
+  hipFree(h->kax2);
+  hipFree(h->nop);
+  hipFree(h->psi_old);
+  hipFree(h->partials);
+  hipFree(h->st);
+  hipFree(h->MUdev);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return GPE_OK;
+}
+
+const char* gpe_last_error(const gpe_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int gpe_set_params(gpe_handle* h, const gpe_params* p) {
+  if (!h || !p) return GPE_EINVAL;
+  if (p->ncomp != 1 && p->ncomp != 2) return fail(h, GPE_EINVAL, "ncomp must be 1 or 2");
+  if (p->n_recalc < 1) return fail(h, GPE_EINVAL, "n_recalc must be >= 1");
+  HIPCHK(h, hipSetDevice(h->device));
+  if (p->ncomp == 2 && !h->psi[1]) {
+    HIPCHK(h, hipMalloc(&h->psi[1], h->npts * sizeof(double2)));
+    HIPCHK(h, hipMalloc(&h->V[1], h->npts * sizeof(double)));
+    HIPCHK(h, hipMemset(h->psi[1], 0, h->npts * sizeof(double2)));
+  }
+  h->prm = *p;
+  h->have_params = true;
+  h->begun = false;
+  return GPE_OK;
+}
+
+int gpe_set_potential(gpe_handle* h, int comp, const double* V) {
+  if (!h || !V || comp < 0 || comp > 1 || !h->V[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipMemcpyAsync(h->V[comp], V, h->npts * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_V[comp] = true;
+  return GPE_OK;
+}
+
+int gpe_set_state(gpe_handle* h, int comp, const double* psi) {
+  if (!h || !psi || comp < 0 || comp > 1 || !h->psi[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  int rc;
+  // a pending scale belongs to the other component as well: materialise first
+  if (h->prm.ncomp == 2 && (rc = materialize(h))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->psi[comp], psi, h->npts * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  s.scale = 1.0;
+  if ((rc = write_state(h, s))) return rc;
+  return GPE_OK;
+}
+
+int gpe_get_state(gpe_handle* h, int comp, double* psi) {
+  if (!h || !psi || comp < 0 || comp > 1 || !h->psi[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  DevState s;
+  int rc;
+  if ((rc = read_state(h, &s))) return rc;
+  HIPCHK(h, hipMemcpyAsync(psi, h->psi[comp], h->npts * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (s.scale != 1.0) {
+    // solve_split.m:72  phi = phi*sqrt(NNN/ncur)
+    for (long long i = 0; i < 2 * h->npts; ++i) psi[i] = psi[i] * s.scale;
+  }
+  return GPE_OK;
+}
+
+int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
+  if (!h || !omega) return GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  if (is_array) {
+    if (!h->cpl) HIPCHK(h, hipMalloc(&h->cpl, h->npts * sizeof(double2)));
+    HIPCHK(h, hipMemcpyAsync(h->cpl, omega, h->npts * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->cpl_array = true;
+  } else {
+    h->cpl_array = false;
+    h->cpl_re = omega[0];
+    h->cpl_im = omega[1];
+  }
+  return GPE_OK;
+}
+
+int gpe_measure(gpe_handle* h, double out[5]) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!out) return GPE_EINVAL;
+  if (h->prm.ncomp == 2) return gpe::spinor_measure(h, out);
+  FinParams f = fin_params(h, gpe::FIN_SUMS);
+  if ((rc = launch_measure(h, 0, &f))) return rc;
+  HIPCHK(h, gpe::launch_finalize(f, h->stream));
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  const double q = s.scale * s.scale;
+  out[0] = q * s.sA * h->w;
+  out[1] = q * s.sB * h->w;
+  out[2] = q * q * s.sC * h->w;
+  out[3] = q * s.sK * h->w;
+  out[4] = 0.0;
+  return GPE_OK;
+}
+
+int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (niter_inner < 1) return fail(h, GPE_EINVAL, "niter_inner must be >= 1");
+  if (h->prm.ncomp == 2) return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
+  if ((rc = materialize(h))) return rc;
+  double m[5];
+  if ((rc = gpe_measure(h, m))) return rc;
+  // solve_split.m:17-21: NN0 = Ntotal, or the norm of the state
+  h->NN0 = h->prm.Ntotal > 0 ? h->prm.Ntotal : m[0];
+  h->n_rec = h->prm.n_recalc;                                  // 24-27
+  if (niter_inner % h->n_rec != 0) h->n_rec = 1;
+  h->niter_inner = niter_inner;
+  h->ddt = ddt;
+  const std::complex<double> dt = cdiv({0.0, ddt}, {1.0, h->prm.gamma});   // 31
+  h->dr = dt.real();
+  h->di = dt.imag();
+  if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;  // 32
+  // 41: muc = real(inner(phi, applyham(phi)))/NN0
+  const double muc = ((m[1] + m[3]) + h->prm.g[0] * m[2]) / h->NN0;
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  s.mu = h->prm.mu_init > 0 ? h->prm.mu_init : muc;             // 42-46
+  s.scale = 1.0;
+  s.NNN = h->NN0;
+  s.muc = muc;
+  s.ncur = h->NN0;
+  if ((rc = write_state(h, s))) return rc;
+  if (muc_out) *muc_out = muc;
+  h->begun = true;
+  return GPE_OK;
+}
+
+int gpe_split_block(gpe_handle* h, double time2, double* muc, double* ncur) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!h->begun) return fail(h, GPE_ESTATE, "gpe_split_begin has not been called");
+  if (h->prm.ncomp == 2) return gpe::spinor_split_block(h, time2, muc, ncur);
+  if ((rc = run_block(h, time2))) return rc;
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  if (muc) *muc = s.muc;
+  if (ncur) *ncur = s.ncur;
+  return GPE_OK;
+}
+
+int gpe_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, double* ncur) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!h->begun) return fail(h, GPE_ESTATE, "gpe_split_begin has not been called");
+  if (h->prm.ncomp == 2) return gpe::spinor_split_outer(h, j, time2_out, muc, ncur);
+  const double dt_outer = h->ddt * h->niter_inner;
+  const double time = (double)(j - 1) * dt_outer;               // solve_split.m:50
+  double time2 = time;
+  const int nblocks = h->niter_inner / h->n_rec;
+  for (int jj = 1; jj <= nblocks; ++jj) {
+    time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
+    if ((rc = run_block(h, time2))) return rc;
+  }
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  if (time2_out) *time2_out = time2;
+  if (muc) *muc = s.muc;
+  if (ncur) *ncur = s.ncur;
+  return GPE_OK;
+}
+
+int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, double* EE, int* n_iter,
+            double* dt_final, int* converged) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!a || !MU || a->max_iter < 1) return fail(h, GPE_EINVAL, "bad ITP arguments");
+  if (h->prm.ncomp != 1) return fail(h, GPE_EINVAL, "gpe_itp supports one component");
+  if ((rc = materialize(h))) return rc;
+  if (!h->nop) HIPCHK(h, hipMalloc(&h->nop, h->npts * sizeof(double)));
+  if (!h->psi_old) HIPCHK(h, hipMalloc(&h->psi_old, h->npts * sizeof(double2)));
+  if (h->MUcap < a->max_iter) {
+    hipFree(h->MUdev);
+    h->MUdev = nullptr;
+    HIPCHK(h, hipMalloc(&h->MUdev, a->max_iter * sizeof(double)));
+    h->MUcap = a->max_iter;
+  }
+  double2* psi = h->psi[0];
+  const double* V = h->V[0];
+  // groundstate_itp.m:28 current_state = phi
+  HIPCHK(h, hipMemcpyAsync(h->psi_old, psi, h->npts * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  s.mu_itp = 1.0;
+  s.scale = 1.0;
+  if ((rc = write_state(h, s))) return rc;
+  double dt = a->dt;
+  const double nnn = h->prm.Ntotal > 0 ? h->prm.Ntotal : 1.0;   // 21-25
+  double dtmin = std::sqrt(a->eps);
+  h->dr = dt;
+  h->di = 0.0;
+  if ((rc = build_kinetic(h, {dt, 0.0}, h->kcol, h->kax))) return rc;   // 30
+  int i = 0;
+  int conv = 1;
+  for (;;) {
+    ++i;
+    // 76 (36 on the first pass) and 40: N from the renormalised density, x N, forward x
+    if ((rc = sweep(h, 0, gpe::S_ITPB, base_params(h, 0, psi, V)))) return rc;
+    if (h->ndims == 3) {
+      if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, V)))) return rc;
+      if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_INV, base_params(h, 1, psi, V)))) return rc;
+    } else {
+      if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
+    }
+    int nb = 0;
+    if ((rc = sweep(h, 0, gpe::S_ITPA, base_params(h, 0, psi, V), &nb))) return rc;   // 41-44
+    FinParams f = fin_params(h, gpe::FIN_ITP);
+    f.cnt[gpe::SL_A] = nb;
+    f.dt = dt;
+    f.Ntotal = h->prm.Ntotal;
+    f.mu_init = h->prm.mu_init;
+    f.MU = h->MUdev;
+    f.iter = i - 1;
+    HIPCHK(h, gpe::launch_finalize(f, h->stream));                // 45-52
+    if (i > a->min_iter && i % 10 == 0) {                          // 54
+      double mu2v = 0.0;
+      if (a->nargout >= 3 || a->nargout >= 4) {
+        // applyham/get_energy of phi = mu_itp * stored state
+        FinParams fm = fin_params(h, gpe::FIN_SUMS);
+        if ((rc = launch_measure(h, 0, &fm))) return rc;
+        HIPCHK(h, gpe::launch_finalize(fm, h->stream));
+        DevState t;
+        if ((rc = read_state(h, &t))) return rc;
+        const double q = t.mu_itp * t.mu_itp;
+        const double base = (q * t.sB + q * t.sK) * h->w;
+        mu2v = base + h->prm.g[0] * (q * q) * t.sC * h->w;
+        if (a->nargout >= 3 && MU2) MU2[i - 1] = mu2v;
+        if (a->nargout >= 4 && EE) EE[i - 1] = base + 0.5 * h->prm.g[0] * (q * q) * t.sC * h->w;
+      }
+      const int eb = elem_blocks(h->npts);
+      HIPCHK(h, gpe::launch_itp_check(psi, h->psi_old, h->st, h->npts, h->partials, h->maxwg, eb, h->stream));
+      FinParams fx = fin_params(h, gpe::FIN_MAX);
+      fx.cnt[gpe::SL_MAX] = eb;
+      fx.dt = dt;
+      HIPCHK(h, gpe::launch_finalize(fx, h->stream));
+      DevState t;
+      if ((rc = read_state(h, &t))) return rc;
+      if (t.delta < a->eps) {                                      // 63
+        if (a->nargout >= 3) {
+          double mui = 0.0;
+          HIPCHK(h, hipMemcpy(&mui, h->MUdev + (i - 1), sizeof(double), hipMemcpyDeviceToHost));
+          dtmin = dt * std::sqrt(a->eps / std::fabs(mui - mu2v / nnn));
+        }
+        if (!a->adjust_stepsize || dt < dtmin) break;
+        dt = dt / 2;
+        h->dr = dt;
+        if ((rc = build_kinetic(h, {dt, 0.0}, h->kcol, h->kax))) return rc;
+      }
+    }
+    if (i >= a->max_iter) {                                        // 78-81
+      conv = 0;
+      break;
+    }
+  }
+  // final phi = mu * stored state (groundstate_itp.m:52)
+  HIPCHK(h, gpe::launch_scale(psi, h->st, 1, h->npts, elem_blocks(h->npts), h->stream));
+  HIPCHK(h, hipMemcpyAsync(MU, h->MUdev, i * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (n_iter) *n_iter = i;
+  if (dt_final) *dt_final = dt;
+  if (converged) *converged = conv;
+  h->begun = false;
+  return GPE_OK;
+}
+
+}  // extern "C"

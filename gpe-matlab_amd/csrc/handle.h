@@ -1,0 +1,310 @@
+// Private definition of the library handle and the helpers shared by the
+// drivers (scalar GPEtask, spinor, adjoint, batch).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpe_hip.h"
+#include "state.h"
+#include "sweep.h"
+
+using gpe::DevState;
+using gpe::FinParams;
+using gpe::SweepParams;
+
+namespace gpe_detail {
+
+struct Axis {
+  int n = 1;
+  long long stride = 1, ncols = 0, A = 1, sa = 0, sb = 0;
+  double2* tw = nullptr;      // forward twiddles
+  double* keax = nullptr;     // k^2/2/n
+  std::vector<double> k;      // wave vectors (grid*/init.m k layout)
+};
+
+}  // namespace gpe_detail
+
+struct gpe_handle {
+  int device = 0;
+  int ndims = 2;
+  int nx = 1, ny = 1, nz = 1;
+  long long npts = 0;
+  double dx = 0, dy = 0, dz = 0, w = 0;
+  gpe_detail::Axis ax[3];  // 0: x, 1: y, 2: z
+  hipStream_t stream = nullptr;
+  double2* psi[2] = {nullptr, nullptr};
+  double* V[2] = {nullptr, nullptr};
+  double2* cpl = nullptr;       // spinor coupling array (or null for scalar)
+  double cpl_re = 0, cpl_im = 0;
+  bool cpl_array = false;
+  double2* kcol = nullptr;      // kinetic factor per mid-axis column
+  double2* kax = nullptr;       // kinetic factor per mid-axis index (incl. 1/npts)
+  double2* kcol2 = nullptr;     // spinor: second kinetic table pair
+  double2* kax2 = nullptr;
+  double* nop = nullptr;        // ITP stored operator
+  double2* psi_old = nullptr;   // ITP convergence reference
+  double* partials = nullptr;
+  int maxwg = 0;
+  DevState* st = nullptr;
+  double* MUdev = nullptr;
+  int MUcap = 0;
+  gpe_params prm{};
+  bool have_params = false;
+  bool have_V[2] = {false, false};
+  // solve_split state
+  bool begun = false;
+  double ddt = 0, dr = 0, di = 0;
+  int niter_inner = 0, n_rec = 1;
+  double NN0 = 0;
+  std::string err;
+};
+
+namespace gpe_detail {
+
+inline int fail(gpe_handle* h, int code, const char* fmt, ...) {
+  if (h) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    h->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(h, x)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fail((h), GPE_EHIP, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+// numpy/MATLAB linspace: x_i = start + i*step, last point set to stop.
+inline std::vector<double> linspace(double a, double b, int n) {
+  std::vector<double> x(n);
+  const double step = (b - a) / (n - 1);
+  for (int i = 0; i < n; ++i) x[i] = (double)i * step + a;
+  x[n - 1] = b;
+  return x;
+}
+
+// grid2d/init.m:27: k = [0:n/2, -(n/2-1):-1] * 2 pi / (x(end) - x(1) + dx)
+inline std::vector<double> kvec(const std::vector<double>& x, double dx) {
+  const int n = (int)x.size();
+  std::vector<double> k(n);
+  const double L = x[n - 1] - x[0] + dx;
+  const double s = 2.0 * M_PI / L;
+  for (int i = 0; i < n; ++i) {
+    const int idx = i <= n / 2 ? i : i - n;
+    k[i] = (double)idx * s;
+  }
+  return k;
+}
+
+// Twiddles e^{-2 pi i j/n} in extended precision, rounded once.
+inline std::vector<double2> twiddles(int n) {
+  std::vector<double2> t(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int j = 0; j < n; ++j) {
+    const long double a = -2.0L * pi * (long double)j / (long double)n;
+    t[j] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return t;
+}
+
+// MATLAB/CPython complex quotient (Smith's algorithm): a / b.
+inline std::complex<double> cdiv(std::complex<double> a, std::complex<double> b) {
+  const double br = b.real(), bi = b.imag();
+  if (std::fabs(br) >= std::fabs(bi)) {
+    const double ratio = bi / br;
+    const double denom = br + bi * ratio;
+    return {(a.real() + a.imag() * ratio) / denom, (a.imag() - a.real() * ratio) / denom};
+  }
+  const double ratio = br / bi;
+  const double denom = br * ratio + bi;
+  return {(a.real() * ratio + a.imag()) / denom, (a.imag() * ratio - a.real()) / denom};
+}
+
+inline int elem_blocks(long long n) {
+  long long nb = (n + 255) / 256;
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  return (int)nb;
+}
+
+inline SweepParams base_params(gpe_handle* h, int axis, double2* psi, const double* V) {
+  SweepParams p{};
+  const Axis& a = h->ax[axis];
+  p.psi = psi;
+  p.V = V;
+  p.tw = a.tw;
+  p.keax = a.keax;
+  p.kcol = h->kcol;
+  p.kax = h->kax;
+  p.nop = h->nop;
+  p.partials = h->partials;
+  p.st = h->st;
+  p.ncols = a.ncols;
+  p.A = a.A;
+  p.sa = a.sa;
+  p.sb = a.sb;
+  p.stride = a.stride;
+  p.maxwg = h->maxwg;
+  p.slot_ke = gpe::SL_KX + axis;
+  p.g = h->prm.g[0];
+  p.dr = h->dr;
+  p.di = h->di;
+  return p;
+}
+
+inline int sweep(gpe_handle* h, int axis, int seq, const SweepParams& p, int* nb = nullptr) {
+  const bool colfast = axis != 1;  // y is contiguous: threads run along the pencil
+  hipError_t e = gpe::launch_sweep(h->ax[axis].n, seq, colfast, p, h->stream, nb);
+  if (e != hipSuccess) return fail(h, GPE_EHIP, "sweep launch (axis %d, seq %d): %s", axis, seq, hipGetErrorString(e));
+  return GPE_OK;
+}
+
+inline int mid_axis(const gpe_handle* h) { return h->ndims == 3 ? 2 : 1; }
+
+// Kinetic tables for complex dt: kcol[col] * kax[i] = exp(-kk dt) / npts
+// (set_kinop.m:9) split into the mid-spectrum axis and its columns.
+inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, double2* kax, double frac = 1.0) {
+  const int ma = mid_axis(h);
+  const Axis& m = h->ax[ma];
+  std::vector<double2> col, axv(m.n);
+  if (h->ndims == 3) {
+    col.resize((size_t)h->nx * h->ny);
+    for (int ix = 0; ix < h->nx; ++ix)
+      for (int iy = 0; iy < h->ny; ++iy) {
+        const double kk = (h->ax[0].k[ix] * h->ax[0].k[ix] + h->ax[1].k[iy] * h->ax[1].k[iy]) / 2;
+        const std::complex<double> e = std::exp(-kk * dt * frac);
+        col[(size_t)iy + (size_t)h->ny * ix] = make_double2(e.real(), e.imag());
+      }
+  } else {
+    col.resize(h->nx);
+    for (int ix = 0; ix < h->nx; ++ix) {
+      const double kk = h->ax[0].k[ix] * h->ax[0].k[ix] / 2;
+      const std::complex<double> e = std::exp(-kk * dt * frac);
+      col[ix] = make_double2(e.real(), e.imag());
+    }
+  }
+  const double inv = 1.0 / (double)h->npts;
+  for (int i = 0; i < m.n; ++i) {
+    const double kk = m.k[i] * m.k[i] / 2;
+    const std::complex<double> e = std::exp(-kk * dt * frac) * inv;
+    axv[i] = make_double2(e.real(), e.imag());
+  }
+  HIPCHK(h, hipMemcpyAsync(kcol, col.data(), col.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(kax, axv.data(), axv.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+inline int read_state(gpe_handle* h, DevState* out) {
+  HIPCHK(h, hipMemcpyAsync(out, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+inline int write_state(gpe_handle* h, const DevState& s) {
+  HIPCHK(h, hipMemcpyAsync(h->st, &s, sizeof(DevState), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+inline FinParams fin_params(gpe_handle* h, int mode) {
+  FinParams f{};
+  f.partials = h->partials;
+  f.maxwg = h->maxwg;
+  f.st = h->st;
+  f.mode = mode;
+  f.w = h->w;
+  f.g = h->prm.g[0];
+  return f;
+}
+
+// Reductions of the stored state: slots A, B, C and the kinetic term of every
+// axis (Parseval per axis: sum_k kk |F psi|^2 / npts = sum_a sum k_a^2/2 |F_a psi|^2 / n_a).
+inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
+  int rc, nb = 0;
+  SweepParams p = base_params(h, 0, h->psi[comp], h->V[comp]);
+  if ((rc = sweep(h, 0, gpe::S_KEX, p, &nb))) return rc;
+  f->cnt[gpe::SL_A] = f->cnt[gpe::SL_B] = f->cnt[gpe::SL_C] = f->cnt[gpe::SL_KX] = nb;
+  for (int a = 1; a < h->ndims; ++a) {
+    SweepParams q = base_params(h, a, h->psi[comp], h->V[comp]);
+    if ((rc = sweep(h, a, gpe::S_KE, q, &nb))) return rc;
+    f->cnt[gpe::SL_KX + a] = nb;
+  }
+  return GPE_OK;
+}
+
+// One block of solve_split.m:53-78 (GPEtask, static or pre-uploaded V).
+inline int run_block(gpe_handle* h, double time2) {
+  int rc;
+  double2* psi = h->psi[0];
+  const double* V = h->V[0];
+  for (int i = 0; i < h->n_rec; ++i) {
+    const int xseq = i == 0 ? gpe::S_OPEN : gpe::S_MID;
+    if (i > 0) {
+      if ((rc = sweep(h, 0, gpe::S_MID, base_params(h, 0, psi, V)))) return rc;
+    } else {
+      if ((rc = sweep(h, 0, xseq, base_params(h, 0, psi, V)))) return rc;
+    }
+    if (h->ndims == 3) {
+      if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, V)))) return rc;
+      if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_INV, base_params(h, 1, psi, V)))) return rc;
+    } else {
+      if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
+    }
+  }
+  if ((rc = sweep(h, 0, gpe::S_CLOSE, base_params(h, 0, psi, V)))) return rc;
+  FinParams f = fin_params(h, gpe::FIN_BLOCK);
+  if ((rc = launch_measure(h, 0, &f))) return rc;
+  f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
+  f.NN0 = h->NN0;
+  f.tau = h->prm.decay_rate;
+  f.time2 = time2;
+  HIPCHK(h, gpe::launch_finalize(f, h->stream));
+  return GPE_OK;
+}
+
+inline int ensure_ready(gpe_handle* h) {
+  if (!h) return GPE_EINVAL;
+  if (!h->have_params) return fail(h, GPE_ESTATE, "gpe_set_params has not been called");
+  for (int c = 0; c < h->prm.ncomp; ++c)
+    if (!h->have_V[c]) return fail(h, GPE_ESTATE, "potential of component %d not set", c);
+  HIPCHK(h, hipSetDevice(h->device));
+  return GPE_OK;
+}
+
+// Apply a pending renormalisation scale to the stored state(s).
+inline int materialize(gpe_handle* h) {
+  DevState s;
+  int rc;
+  if ((rc = read_state(h, &s))) return rc;
+  if (s.scale != 1.0) {
+    for (int c = 0; c < h->prm.ncomp; ++c) {
+      HIPCHK(h, gpe::launch_scale(h->psi[c], h->st, 0, h->npts, elem_blocks(h->npts), h->stream));
+      if (c + 1 < h->prm.ncomp) {
+        // launch_scale resets the scale; restore it for the next component
+        DevState t;
+        if ((rc = read_state(h, &t))) return rc;
+        t.scale = s.scale;
+        if ((rc = write_state(h, t))) return rc;
+      }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return GPE_OK;
+}
+
+}  // namespace gpe_detail

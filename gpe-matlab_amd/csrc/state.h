@@ -1,0 +1,31 @@
+// Block-boundary kernels: second-stage reductions and device-side scalars.
+#pragma once
+#include "sweep.h"
+
+namespace gpe {
+
+enum FinMode : int { FIN_SUMS = 0, FIN_BLOCK = 1, FIN_ITP = 2, FIN_MAX = 3 };
+
+struct FinParams {
+  const double* partials;
+  int maxwg;
+  int cnt[SL_NSLOT];   // number of partials in each slot (0 = unused)
+  DevState* st;
+  int mode;
+  int renorm;          // FIN_BLOCK: gamma > 0 && mu_init == 0
+  double w;            // grid weight dx*dy(*dz)
+  double g;
+  double NN0, tau, time2;
+  double dt;           // FIN_ITP / FIN_MAX: real time step
+  double Ntotal, mu_init;
+  double* MU;          // FIN_ITP: device history
+  int iter;
+};
+
+hipError_t launch_finalize(const FinParams& f, hipStream_t s);
+hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st, long long n, double* partials,
+                            int maxwg, int nblocks, hipStream_t s);
+hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s);
+int sweep_cols_per_block(int n);
+
+}  // namespace gpe

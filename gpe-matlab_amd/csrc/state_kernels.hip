@@ -1,0 +1,168 @@
+// Block-boundary kernels: second-stage reductions, device-side scalar updates
+// (norm, renormalisation, chemical potential) and small elementwise passes.
+#include "fft_dev.h"
+#include "reduce_dev.h"
+#include "state.h"
+
+namespace gpe {
+
+// Sum `count` partials of one slot in a fixed order (valid on thread 0).
+__device__ double slot_sum(const double* part, int maxwg, int slot, int count, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) acc += part[slot * maxwg + i];
+  return block_sum(acc, red);
+}
+
+__device__ double slot_max(const double* part, int maxwg, int slot, int count, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) acc = fmax(acc, part[slot * maxwg + i]);
+  return block_max(acc, red);
+}
+
+__global__ void __launch_bounds__(256) finalize_kernel(const FinParams f) {
+  __shared__ double red[4];
+  DevState* st = f.st;
+  if (f.mode == FIN_MAX) {
+    const double m = slot_max(f.partials, f.maxwg, SL_MAX, f.cnt[SL_MAX], red);
+    if (threadIdx.x == 0) st->delta = m / (f.dt * 10.0);   // groundstate_itp.m:61
+    return;
+  }
+  const double sA = f.cnt[SL_A] ? slot_sum(f.partials, f.maxwg, SL_A, f.cnt[SL_A], red) : 0.0;
+  const double sB = f.cnt[SL_B] ? slot_sum(f.partials, f.maxwg, SL_B, f.cnt[SL_B], red) : 0.0;
+  const double sC = f.cnt[SL_C] ? slot_sum(f.partials, f.maxwg, SL_C, f.cnt[SL_C], red) : 0.0;
+  const double kx = f.cnt[SL_KX] ? slot_sum(f.partials, f.maxwg, SL_KX, f.cnt[SL_KX], red) : 0.0;
+  const double ky = f.cnt[SL_KY] ? slot_sum(f.partials, f.maxwg, SL_KY, f.cnt[SL_KY], red) : 0.0;
+  const double kz = f.cnt[SL_KZ] ? slot_sum(f.partials, f.maxwg, SL_KZ, f.cnt[SL_KZ], red) : 0.0;
+  if (threadIdx.x != 0) return;
+  const double sK = (kx + ky) + kz;
+  st->sA = sA; st->sB = sB; st->sC = sC; st->sK = sK;
+  const double w = f.w;
+  if (f.mode == FIN_BLOCK) {
+    // solve_split.m:65-78
+    const double ncur = sA * w;
+    double muc;
+    if (f.renorm) {
+      double NNN = st->NNN;
+      if (f.tau > 0) NNN = f.NN0 * exp(-f.time2 / f.tau);
+      const double q = NNN / ncur;
+      st->scale = sqrt(q);
+      // real(inner(phi_s, applyham(phi_s)))/NNN with phi_s = sqrt(q) phi
+      muc = ((q * sB + q * sK) + f.g * (q * q) * sC) * w / NNN;
+      st->mu = muc;
+      st->NNN = NNN;
+    } else {
+      st->scale = 1.0;
+      muc = ((sB + sK) + f.g * sC) * w / ncur;
+    }
+    st->muc = muc;
+    st->ncur = ncur;
+  } else if (f.mode == FIN_ITP) {
+    // groundstate_itp.m:44-52
+    double mu;
+    if (f.Ntotal > 0) {
+      mu = sqrt(f.Ntotal / (sA * w));
+      if (f.MU) f.MU[f.iter] = log(mu) / f.dt;
+    } else {
+      mu = exp(f.mu_init * f.dt);
+      if (f.MU) f.MU[f.iter] = (sA * w) * (mu * mu);
+    }
+    st->mu_itp = mu;
+  }
+}
+
+// groundstate_itp.m:61-62: max | |s psi| - |psi_old| |, then psi_old = s psi.
+__global__ void __launch_bounds__(256) itp_check_kernel(const double2* __restrict__ psi, double2* __restrict__ old,
+                                                        const DevState* st, long long n, double* partials,
+                                                        int maxwg) {
+  __shared__ double red[4];
+  const double s = st->mu_itp;
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double2 x = cscale(psi[i], s);
+    const double2 o = old[i];
+    m = fmax(m, fabs(hypot(x.x, x.y) - hypot(o.x, o.y)));
+    old[i] = x;
+  }
+  const double r = block_max(m, red);
+  if (threadIdx.x == 0) partials[SL_MAX * maxwg + blockIdx.x] = r;
+}
+
+// psi *= s (s read from the device state: 0 -> scale, 1 -> mu_itp); resets the
+// pending scale when it materialises it.
+__global__ void __launch_bounds__(256) scale_kernel(double2* __restrict__ psi, DevState* st, int which,
+                                                    long long n) {
+  const double s = which == 0 ? st->scale : st->mu_itp;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    psi[i] = cscale(psi[i], s);
+}
+
+__global__ void reset_scale_kernel(DevState* st, int which) {
+  if (threadIdx.x == 0) {
+    if (which == 0) st->scale = 1.0; else st->mu_itp = 1.0;
+  }
+}
+
+hipError_t launch_finalize(const FinParams& f, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st, long long n, double* partials,
+                            int maxwg, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(itp_check_kernel, dim3(nblocks), dim3(256), 0, s, psi, old, st, n, partials, maxwg);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(scale_kernel, dim3(nblocks), dim3(256), 0, s, psi, st, which, n);
+  hipLaunchKernelGGL(reset_scale_kernel, dim3(1), dim3(64), 0, s, st, which);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ length dispatch
+#define GPE_DECL(N) hipError_t launch_sweep_##N(int, bool, const SweepParams&, hipStream_t, int*);
+GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
+GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
+#undef GPE_DECL
+
+bool sweep_length_supported(int n) {
+  switch (n) {
+    case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
+    case 384: case 512:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int sweep_cols_per_block(int n) {
+  switch (n) {
+    case 8: return 256 / Plan<8>::T;
+    case 12: return 256 / Plan<12>::T;
+    case 16: return 256 / Plan<16>::T;
+    case 24: return 256 / Plan<24>::T;
+    case 32: return 256 / Plan<32>::T;
+    case 48: return 256 / Plan<48>::T;
+    case 64: return 256 / Plan<64>::T;
+    case 96: return 256 / Plan<96>::T;
+    case 128: return 256 / Plan<128>::T;
+    case 192: return 256 / Plan<192>::T;
+    case 256: return 256 / Plan<256>::T;
+    case 384: return 256 / Plan<384>::T;
+    case 512: return 256 / Plan<512>::T;
+    default: return 0;
+  }
+}
+
+hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks) {
+  switch (n) {
+#define GPE_C(N) case N: return launch_sweep_##N(seq, colfast, p, s, nblocks);
+    GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
+    GPE_C(256) GPE_C(384) GPE_C(512)
+#undef GPE_C
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gpe

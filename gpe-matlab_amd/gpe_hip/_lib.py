@@ -1,0 +1,218 @@
+"""ctypes binding of libgpehip.so (declarations in include/gpe_hip.h).
+
+There is no CPU fallback: if the library is missing or fails to load, every
+solver call raises ``GPELibraryError``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.environ.get("GPE_HIP_LIB", os.path.join(HERE, "libgpehip.so"))
+
+GPE_OK = 0
+ERRNAMES = {-1: "GPE_EINVAL", -2: "GPE_EHIP", -3: "GPE_ENOMEM", -4: "GPE_ESTATE", -5: "GPE_ENONFINITE"}
+
+
+class GPELibraryError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("ndims", C.c_int), ("n", C.c_int * 3), ("xmax", C.c_double * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("ncomp", C.c_int), ("g", C.c_double * 4), ("gamma", C.c_double), ("decay_rate", C.c_double),
+                ("Ntotal", C.c_double), ("mu_init", C.c_double), ("n_recalc", C.c_int)]
+
+
+class ItpArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("min_iter", C.c_int),
+                ("adjust_stepsize", C.c_int), ("nargout", C.c_int)]
+
+
+# name -> (argtypes); every entry point returns int
+_P = C.c_void_p
+_D = C.c_double
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+SIGNATURES = {
+    "gpe_create": [C.POINTER(GridDesc), C.POINTER(C.c_void_p)],
+    "gpe_destroy": [_P],
+    "gpe_set_params": [_P, C.POINTER(Params)],
+    "gpe_set_potential": [_P, C.c_int, _DP],
+    "gpe_set_state": [_P, C.c_int, _DP],
+    "gpe_get_state": [_P, C.c_int, _DP],
+    "gpe_set_coupling": [_P, _DP, C.c_int],
+    "gpe_split_begin": [_P, _D, C.c_int, _DP],
+    "gpe_split_outer": [_P, C.c_int, _DP, _DP, _DP],
+    "gpe_split_block": [_P, _D, _DP, _DP],
+    "gpe_measure": [_P, _DP],
+    "gpe_itp": [_P, C.POINTER(ItpArgs), _DP, _DP, _DP, _IP, _DP, _IP],
+    "gpe_device_info": [C.c_char_p, C.c_int],
+}
+OPTIONAL = set()
+
+_lib = None
+
+
+def load():
+    """Load the HIP library once; raise if it is absent (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise GPELibraryError(
+            f"{LIBPATH} not found: build it with `python gpe-matlab_amd/build.py` "
+            "(the split-step solvers have no CPU fallback)")
+    lib = C.CDLL(LIBPATH)
+    for name, args in SIGNATURES.items():
+        try:
+            f = getattr(lib, name)
+        except AttributeError:
+            if name in OPTIONAL:
+                continue
+            raise GPELibraryError(f"{LIBPATH} does not export {name}")
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.gpe_last_error.argtypes = [_P]
+    lib.gpe_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def check(rc, h=None, what=""):
+    if rc != GPE_OK:
+        msg = ""
+        if h is not None and _lib is not None:
+            m = _lib.gpe_last_error(h)
+            msg = m.decode() if m else ""
+        raise GPELibraryError(f"{what}: {ERRNAMES.get(rc, rc)} {msg}")
+
+
+def to_colmajor_c128(a):
+    """MATLAB [ny, nx(, nz)] array -> contiguous column-major interleaved complex128."""
+    a = np.asarray(a, dtype=np.complex128)
+    return np.ascontiguousarray(a.T).reshape(-1)
+
+
+def to_colmajor_f64(a):
+    a = np.asarray(a, dtype=np.float64)
+    return np.ascontiguousarray(a.T).reshape(-1)
+
+
+def from_colmajor_c128(flat, shape):
+    return flat.reshape(tuple(reversed(shape))).T.copy(order="C")
+
+
+class Device:
+    """Owns one gpe_handle (device buffers, tables, stream) for one grid."""
+
+    def __init__(self, grid):
+        self.lib = load()
+        d = GridDesc()
+        d.ndims = grid.ndims
+        n = [grid.nx, grid.ny, grid.nz if grid.ndims == 3 else 1]
+        xm = [grid.xmax, grid.ymax, grid.zmax if grid.ndims == 3 else 1.0]
+        for i in range(3):
+            d.n[i] = int(n[i])
+            d.xmax[i] = float(xm[i])
+        h = C.c_void_p()
+        rc = self.lib.gpe_create(C.byref(d), C.byref(h))
+        if rc != GPE_OK:
+            raise GPELibraryError(f"gpe_create failed ({ERRNAMES.get(rc, rc)}) for grid n={n}")
+        self.h = h
+        self.shape = grid.size
+        self.npts = int(np.prod(grid.size))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gpe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        check(getattr(self.lib, name)(self.h, *args), self.h, name)
+
+    def set_params(self, ncomp, g, gamma, decay_rate, Ntotal, mu_init, n_recalc):
+        p = Params()
+        p.ncomp = ncomp
+        gm = np.asarray(g, dtype=np.float64)
+        if ncomp == 1:
+            p.g[0] = float(gm.ravel()[0])
+        else:
+            flat = gm.reshape(2, 2).T.ravel()  # column-major {g11, g21, g12, g22}
+            for i in range(4):
+                p.g[i] = float(flat[i])
+        p.gamma, p.decay_rate, p.Ntotal, p.mu_init = float(gamma), float(decay_rate), float(Ntotal), float(mu_init)
+        p.n_recalc = int(n_recalc)
+        self.call("gpe_set_params", C.byref(p))
+
+    def set_potential(self, comp, V):
+        v = to_colmajor_f64(np.broadcast_to(V, self.shape))
+        self.call("gpe_set_potential", comp, dptr(v))
+
+    def set_state(self, comp, psi):
+        v = to_colmajor_c128(np.broadcast_to(psi, self.shape))
+        self.call("gpe_set_state", comp, v.view(np.float64).ctypes.data_as(_DP))
+
+    def get_state(self, comp=0):
+        out = np.empty(self.npts, dtype=np.complex128)
+        self.call("gpe_get_state", comp, out.view(np.float64).ctypes.data_as(_DP))
+        return from_colmajor_c128(out, self.shape)
+
+    def set_coupling(self, omega):
+        if np.isscalar(omega) or np.ndim(omega) == 0:
+            v = np.array([np.real(omega), np.imag(omega)], dtype=np.float64)
+            self.call("gpe_set_coupling", dptr(v), 0)
+        else:
+            v = to_colmajor_c128(np.broadcast_to(omega, self.shape))
+            self.call("gpe_set_coupling", v.view(np.float64).ctypes.data_as(_DP), 1)
+
+    def measure(self):
+        out = np.zeros(5)
+        self.call("gpe_measure", dptr(out))
+        return out
+
+    def split_begin(self, ddt, niter_inner):
+        muc = C.c_double()
+        self.call("gpe_split_begin", float(ddt), int(niter_inner), C.byref(muc))
+        return muc.value
+
+    def split_outer(self, j):
+        t, m, n = C.c_double(), C.c_double(), C.c_double()
+        self.call("gpe_split_outer", int(j), C.byref(t), C.byref(m), C.byref(n))
+        return t.value, m.value, n.value
+
+    def split_block(self, time2):
+        m, n = C.c_double(), C.c_double()
+        self.call("gpe_split_block", float(time2), C.byref(m), C.byref(n))
+        return m.value, n.value
+
+    def itp(self, dt, eps, max_iter, min_iter, adjust, nargout):
+        a = ItpArgs(float(dt), float(eps), int(max_iter), int(min_iter), int(bool(adjust)), int(nargout))
+        MU = np.zeros(max_iter)
+        MU2 = np.zeros(max_iter)
+        EE = np.zeros(max_iter)
+        nit, dtf, conv = C.c_int(), C.c_double(), C.c_int()
+        self.call("gpe_itp", C.byref(a), dptr(MU), dptr(MU2), dptr(EE), C.byref(nit), C.byref(dtf), C.byref(conv))
+        i = nit.value
+        return MU[:i], MU2[:i], EE[:i], i, dtf.value, bool(conv.value)
+
+
+def device_info():
+    lib = load()
+    buf = C.create_string_buffer(256)
+    lib.gpe_device_info(buf, 256)
+    return buf.value.decode()

@@ -1,0 +1,279 @@
+"""GPEtask / GPEtaskSpinor with the reference's API, running on the HIP library.
+
+``task.solve_split(ddt, niter_inner, niter_outer)`` and
+``task.groundstate_itp(dt, eps, phi0)`` keep the reference signatures, property
+names, side effects (current_state/current_iter/current_time/history, user
+callback) and resume semantics; their bodies call the C ABI (include/gpe_hip.h),
+which runs every step on the GPU.  There is no CPU fallback: without the built
+library every solver call raises ``GPELibraryError``.
+
+Reference: lib/@GPEtask/GPEtask.m, solve_split.m, groundstate_itp.m,
+groundstate_tf.m; lib/@GPEtaskSpinor/GPEtaskSpinor.m, solve_split.m.
+"""
+import warnings
+
+import numpy as np
+
+from . import _lib
+
+_UNSET = object()
+
+
+class GPEtask:
+    """Drop-in for lib/@GPEtask (GPEtask.m:1-167), omega == 0."""
+
+    _ncomp = 1
+
+    def __init__(self, grid, trappot, seed=None):
+        # GPEtask.m:41-60
+        self.grid = grid
+        self.g = 0.0
+        self.omega = 0.0
+        self.n_crank = 3
+        self.n_recalc = 10
+        self.gamma = 0.0
+        self.decay_rate = 0.0
+        self.Ntotal = 0.0
+        self.mu_init = 0.0
+        self.Vtd = None
+        self.V0 = 0
+        self.Vcurrent = 0
+        self.init_state = np.zeros(grid.size, dtype=np.complex128)
+        self.current_time = 0.0
+        self.current_iter = 0
+        self.current_mu = 0.0
+        self.current_n = None
+        self.history = {"mu": np.zeros(0), "n": np.zeros(0)}
+        self.user_callback = None
+        self.itp_max_iter = 5000
+        self.itp_min_iter = 50
+        self.itp_adjust_stepsize = False
+        self.show_image = 0
+        if callable(trappot):
+            self.Vtrap = trappot(grid.mesh.x, grid.mesh.y, grid.mesh.z)
+        else:
+            self.Vtrap = trappot
+        # GPEtask.m:49 seeds from the clock (rng('shuffle')); a seed makes it reproducible
+        self.rng = np.random.Generator(np.random.PCG64(seed))
+        self._dev = None
+        self._host_state = None      # host copy of current_state
+        self._dev_state_valid = False  # device holds current_state (maybe newer than host copy)
+        self._host_stale = False
+
+    # ------------------------------------------------------------------ state
+    def _device(self):
+        if self._dev is None:
+            self._dev = _lib.Device(self.grid)
+        return self._dev
+
+    @property
+    def current_state(self):
+        if self._host_stale:
+            self._host_state = self._fetch()
+            self._host_stale = False
+        return self._host_state
+
+    @current_state.setter
+    def current_state(self, value):
+        self._host_state = value
+        self._host_stale = False
+        self._dev_state_valid = False
+
+    def _fetch(self):
+        return self._device().get_state(0)
+
+    def _upload(self, phi):
+        self._device().set_state(0, phi)
+
+    def getVtotal(self, time):
+        # GPEtask.m:62-71
+        if time == 0 and np.size(self.V0) > 1:
+            return self.V0
+        if callable(self.Vtd):
+            return self.Vtrap + self.Vtd(self, time)
+        return self.Vtrap
+
+    def _check_omega(self):
+        if not np.isscalar(self.omega) or self.omega != 0:
+            raise NotImplementedError("rotating frame (omega != 0) is not part of the accelerated path")
+
+    def _set_params(self, Ntotal_eff):
+        self._device().set_params(self._ncomp, self.g, self.gamma, self.decay_rate, Ntotal_eff, self.mu_init,
+                                  self.n_recalc)
+
+    def _hist_set(self, key, step, val):
+        arr = self.history[key]
+        if step > arr.size:
+            arr = np.concatenate([arr, np.zeros(step - arr.size, dtype=np.result_type(arr, val))])
+        elif np.iscomplexobj(val) and not np.iscomplexobj(arr):
+            arr = arr.astype(np.complex128)
+        arr[step - 1] = val
+        self.history[key] = arr
+
+    def ext_callback(self, step, time, mu, n):
+        # GPEtask.m:115-142: the state stays on the device and is fetched on first access
+        self._host_stale = True
+        self._dev_state_valid = True
+        self.current_time = time
+        self.current_iter = step
+        self.current_mu = mu
+        self._hist_set("mu", step, mu)
+        self.current_n = n
+        self._hist_set("n", step, n)
+        res_text = ""
+        if callable(self.user_callback):
+            r = self.user_callback(self)
+            res_text = r if isinstance(r, str) else ""
+        return res_text
+
+    # ------------------------------------------------------------------ energies
+    def get_energy(self, phi=None, time=None):
+        """GPEtask.m:97-107 -- real <phi|H_{g/2} phi>, reduced on the device."""
+        m = self._measure(phi, time)
+        return (m[1] + m[3]) + 0.5 * self.g * m[2]
+
+    def chemical_potential_integral(self, phi=None, time=None):
+        """real(grid.inner(phi, applyham(phi))) (GPEtask.m:73-78 + grid1d/inner.m:12)."""
+        m = self._measure(phi, time)
+        return (m[1] + m[3]) + self.g * m[2]
+
+    def _measure(self, phi, time):
+        self._check_omega()
+        dev = self._device()
+        self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+        if time is None:
+            time = self.current_time
+        dev.set_potential(0, self.getVtotal(time))
+        if phi is None:
+            if not self._dev_state_valid:
+                self._upload(self.current_state)
+                self._dev_state_valid = True
+            return dev.measure()
+        saved = self._fetch() if self._dev_state_valid else None
+        self._upload(phi)
+        m = dev.measure()
+        if saved is not None:
+            self._upload(saved)
+        return m
+
+    # ------------------------------------------------------------------ solvers
+    def groundstate_tf(self, eps, N=None, V=None, mul=None, mur=None):
+        """groundstate_tf.m:1-37 -- one-off host bisection for the Thomas-Fermi profile."""
+        if mul is None or mur is None:
+            if V is None:
+                if N is None:
+                    N = self.Ntotal
+                V = self.getVtotal(0)
+            mul = np.min(V)
+            mur = np.max(V)
+        g1 = float(np.ravel(self.g)[0])
+        NC = N * g1
+        mu = (mul + mur) / 2
+        vvv = None
+        while (mur - mul) / mu > eps:
+            vvv = (mu - V) * (V < mu)
+            NN = self.grid.integrate(vvv)
+            if NN > NC:
+                mur = mu
+            else:
+                mul = mu
+            mu = (mul + mur) / 2
+        return np.sqrt(vvv / g1), mu
+
+    def process_init_state(self, phi0):
+        # GPEtask.m:144-164
+        if isinstance(phi0, str):
+            if self.Ntotal > 0:
+                if phi0 == "tf":
+                    res, _ = self.groundstate_tf(np.finfo(np.float64).eps)
+                else:
+                    r = self.rng.random(self.grid.size) + 1j * self.rng.random(self.grid.size)
+                    res = np.sqrt(self.Ntotal) * self.grid.normalize(r)
+            else:
+                V = self.getVtotal(0)
+                res = np.real(np.sqrt((self.mu_init - V + 0j) / self.g))
+            return res
+        if self.Ntotal > 0:
+            return np.sqrt(self.Ntotal) * self.grid.normalize(phi0)
+        return phi0
+
+    def _static_potential(self):
+        return not callable(self.Vtd) and np.size(self.V0) <= 1
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """solve_split.m:1-84 on the GPU; returns the final state."""
+        self._check_omega()
+        dev = self._device()
+        start = self.current_iter
+        if self.Ntotal > 0:                                     # solve_split.m:17-21
+            NN0 = self.Ntotal
+        else:
+            self._set_params(1.0)
+            dev.set_potential(0, self.getVtotal(self.current_time))
+            self._upload(self.init_state)
+            self._dev_state_valid = False
+            NN0 = dev.measure()[0]
+        self._set_params(NN0)
+        if start > 0:                                           # 34-38
+            if not self._dev_state_valid:
+                self._upload(self.current_state)
+        else:
+            self._upload(self.init_state)
+        self._dev_state_valid = True
+        Vb = self.getVtotal(self.current_time)                 # applyham(phi) at line 41
+        dev.set_potential(0, Vb)
+        dev.split_begin(ddt, niter_inner)
+        static = self._static_potential()
+        n_rec = self.n_recalc if niter_inner % self.n_recalc == 0 else 1
+        if static and Vb is not self.Vtrap:
+            dev.set_potential(0, self.Vtrap)
+        dt_outer = ddt * niter_inner
+        for j in range(start + 1, niter_outer + 1):             # 49
+            if static:
+                time2, muc, ncur = dev.split_outer(j)
+            else:
+                time = (j - 1) * dt_outer
+                for jj in range(1, niter_inner // n_rec + 1):
+                    time2 = time + (jj - 1) * ddt * n_rec
+                    dev.set_potential(0, self.getVtotal(time2))   # 54
+                    muc, ncur = dev.split_block(time2)
+            self.ext_callback(j, time2, muc, ncur)              # 80
+        return self.current_state
+
+    def groundstate_itp(self, dt, eps, phi0="rand", nargout=1):
+        """groundstate_itp.m:1-112 on the GPU.  ``nargout`` plays MATLAB's nargout."""
+        self._check_omega()
+        dev = self._device()
+        V = self.getVtotal(0)                                   # 20
+        phi = self.process_init_state(phi0)                     # 27
+        self.current_state = phi                                # 28
+        self._set_params(self.Ntotal)
+        dev.set_potential(0, V)
+        self._upload(phi)
+        MU, MU2, EE, i, dt_end, conv = dev.itp(dt, eps, self.itp_max_iter, self.itp_min_iter,
+                                               self.itp_adjust_stepsize, nargout)
+        if not conv:
+            warnings.warn("Convergence not reached")           # 78-81
+        self._dev_state_valid = True
+        self._host_stale = True
+        phi = self.current_state
+        self.itp_iterations = i
+        out = [phi]
+        if nargout >= 2:                                        # 84-91
+            if self.Ntotal > 0:
+                self.current_mu = MU[-1]
+                self.current_n = self.Ntotal
+            out.append(MU)
+        if nargout >= 3:                                        # 92-100
+            if self.Ntotal > 0:
+                MU2 = MU2 / self.Ntotal
+            else:
+                MU2 = MU2 / MU
+                self.current_mu = MU2[-1]
+                self.current_n = MU[-1]
+            out.append(MU2)
+        if nargout >= 4:                                        # 101-108
+            EE = EE / self.Ntotal if self.Ntotal > 0 else EE / MU
+            out.append(EE)
+        self.init_state = phi                                   # 111
+        return out[0] if nargout == 1 else tuple(out)

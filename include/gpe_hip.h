@@ -1,0 +1,122 @@
+/*
+ * gpe_hip.h -- C ABI of the MI355X (gfx950) split-step Fourier library.
+ *
+ * The reference (GPE-MATLAB) has no FFI: its boundary is MATLAB method dispatch on
+ * handle classes.  These entry points are exactly what a MEX gateway behind those
+ * methods binds; each comment names the reference method (file:line) whose body
+ * the call replaces.  Plain pointers and sizes only; no exceptions cross the ABI.
+ *
+ * Conventions
+ *   - Arrays use MATLAB's column-major layout [ny, nx(, nz)] (y contiguous), the
+ *     layout of grid.mesh.* (grid2d/init.m:34, grid3d/init.m:41).
+ *   - Complex arrays are interleaved complex128 (re, im), as MATLAB R2018a+
+ *     mxGetComplexDoubles hands them out.
+ *   - Every call returns 0 on success and a negative GPE_E* code on failure;
+ *     gpe_last_error(h) then holds a message.  A handle is not thread-safe; calls
+ *     are synchronous at return.
+ *   - Supported axis lengths: 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512.
+ */
+#ifndef GPE_HIP_H
+#define GPE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPE_OK 0
+#define GPE_EINVAL (-1)   /* bad argument / unsupported size */
+#define GPE_EHIP (-2)     /* HIP runtime error */
+#define GPE_ENOMEM (-3)   /* device allocation failed */
+#define GPE_ESTATE (-4)   /* call out of order (e.g. outer step before begin) */
+#define GPE_ENONFINITE (-5) /* a block reduction produced NaN/Inf */
+
+typedef struct gpe_handle gpe_handle;
+
+/* Grid: grid2d(xmax,nx,ymax,ny) / grid3d(xmax,nx,ymax,ny,zmax,nz)
+ * (lib/@grid2d/init.m:5-37, lib/@grid3d/init.m:5-46).  n = {nx, ny, nz},
+ * xmax = {xmax, ymax, zmax}; for ndims == 2, n[2] and xmax[2] are ignored. */
+typedef struct {
+  int ndims;
+  int n[3];
+  double xmax[3];
+} gpe_grid_desc;
+
+/* Task properties read by the hot path (lib/@GPEtask/GPEtask.m:4-34;
+ * GPEtaskSpinor adds the 2x2 g matrix, lib/@GPEtaskSpinor/GPEtaskSpinor.m:5-9). */
+typedef struct {
+  int ncomp;          /* 1: GPEtask, 2: GPEtaskSpinor                              */
+  double g[4];        /* g, or the 2x2 matrix column-major {g11, g21, g12, g22}    */
+  double gamma;       /* GPEtask.m:10 */
+  double decay_rate;  /* GPEtask.m:11 */
+  double Ntotal;      /* GPEtask.m:12 */
+  double mu_init;     /* GPEtask.m:13 */
+  int n_recalc;       /* GPEtask.m:9  */
+} gpe_params;
+
+/* grid2d/grid3d construction + GPEtask(grid, V) (GPEtask.m:41-60): allocates the
+ * device state, twiddle and k tables.  Selects device 0 unless GPE_DEVICE is set. */
+int gpe_create(const gpe_grid_desc* desc, gpe_handle** out);
+int gpe_destroy(gpe_handle* h);
+const char* gpe_last_error(const gpe_handle* h);
+
+/* Property setters (GPEtask.m:4-34). */
+int gpe_set_params(gpe_handle* h, const gpe_params* p);
+
+/* Vtrap / getVtotal(time) result for component comp (GPEtask.m:44-48, 62-71;
+ * GPEtaskSpinor.m:24-44).  V is real float64, npts values. */
+int gpe_set_potential(gpe_handle* h, int comp, const double* V);
+
+/* init_state / current_state upload and download (GPEtask.m:18-19, 119).  The
+ * download applies a pending renormalisation (solve_split.m:72). */
+int gpe_set_state(gpe_handle* h, int comp, const double* psi);
+int gpe_get_state(gpe_handle* h, int comp, double* psi);
+
+/* Spinor Rabi coupling (GPEtaskSpinor.m:6): scalar (is_array = 0, omega = {re, im})
+ * or space-dependent complex array of npts values (is_array = 1). */
+int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array);
+
+/* solve_split prologue (lib/@GPEtask/solve_split.m:13-46;
+ * lib/@GPEtaskSpinor/solve_split.m:13-57): NN0, n_rec fallback, complex dt,
+ * kinetic operator, initial chemical potential.  muc_out receives muc (the
+ * value the reference computes at line 41). */
+int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out);
+
+/* One iteration j of the outer loop with a static potential
+ * (solve_split.m:49-79; spinor solve_split.m:60-115): runs niter_inner/n_rec
+ * blocks and returns the values passed to ext_callback (solve_split.m:80):
+ * time2 (start time of the last block), muc and ncur. */
+int gpe_split_outer(gpe_handle* h, int j, double* time2, double* muc, double* ncur);
+
+/* One block (solve_split.m:53-78) at block time time2 with whatever potential is
+ * currently uploaded; used when Vtd is a host function (GPEtask.m:65-67). */
+int gpe_split_block(gpe_handle* h, double time2, double* muc, double* ncur);
+
+/* Reductions behind applyham/get_energy on the current state (GPEtask.m:73-107):
+ * out = { sum|psi|^2 w, sum V|psi|^2 w, sum |psi|^4 w, Re<psi|lap psi> }
+ * (spinor: summed over components; out[4] = 2 Re sum conj(psi1) Omega psi2 w). */
+int gpe_measure(gpe_handle* h, double out[5]);
+
+/* groundstate_itp after process_init_state (lib/@GPEtask/groundstate_itp.m:20-112).
+ * The initial state must be uploaded (gpe_set_state).  MU, MU2, EE are caller
+ * arrays of max_iter entries (MU2/EE may be NULL when nargout < 3/4).  Returns
+ * the iteration count in n_iter, the final dt (itp_adjust_stepsize halves it),
+ * and converged = 0 when itp_max_iter stopped the loop (the reference warns). */
+typedef struct {
+  double dt;
+  double eps;
+  int max_iter;          /* itp_max_iter, GPEtask.m:26 */
+  int min_iter;          /* itp_min_iter, GPEtask.m:27 */
+  int adjust_stepsize;   /* itp_adjust_stepsize, GPEtask.m:29 */
+  int nargout;           /* 1..4, as MATLAB's nargout */
+} gpe_itp_args;
+int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, double* EE, int* n_iter,
+            double* dt_final, int* converged);
+
+/* Library/device information for diagnostics: writes a NUL-terminated string. */
+int gpe_device_info(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPE_HIP_H */

@@ -1,0 +1,456 @@
+"""Oracle solvers: restatement of lib/@GPEtask and lib/@GPEtaskSpinor (omega = 0 paths).
+
+Test infrastructure only (see ``oracle/__init__.py``).  The control flow, the order
+of the pointwise operations and the block structure follow the MATLAB files line by
+line; the cited line numbers are those of the reference files.
+
+Deviations, all forced by the environment and stated here:
+  * ``rng('shuffle')`` (GPEtask.m:49) is replaced by a seeded numpy PCG64 generator
+    (``task.seed``), so random initial guesses are reproducible.
+  * ``dispstat``/``toc``/``imagesc`` console and plot output is not reproduced.
+  * Only ``omega == 0`` (no rotating frame) is restated; other values raise.
+"""
+import numpy as np
+
+
+def _matlab_eps():
+    return np.finfo(np.float64).eps
+
+
+class GPEtask:
+    """GPEtask.m:1-167 (handle class; Python objects are references as well)."""
+
+    def __init__(self, grid, trappot, seed=0):
+        # GPEtask.m:41-60
+        self.grid = grid
+        self.g = 0.0
+        self.omega = 0.0
+        self.n_crank = 3
+        self.n_recalc = 10
+        self.gamma = 0.0
+        self.decay_rate = 0.0
+        self.Ntotal = 0.0
+        self.mu_init = 0.0
+        self.Vtd = None
+        self.V0 = 0
+        self.current_state = None
+        self.current_time = 0.0
+        self.current_iter = 0
+        self.current_mu = 0.0
+        self.current_n = None
+        self.user_callback = None
+        self.itp_max_iter = 5000
+        self.itp_min_iter = 50
+        self.itp_adjust_stepsize = False
+        self.show_image = 0
+        self.init_state = np.zeros(grid.size, dtype=np.complex128)   # GPEtask.m:43
+        if callable(trappot):                                         # GPEtask.m:44-48
+            self.Vtrap = trappot(grid.mesh.x, grid.mesh.y, grid.mesh.z)
+        else:
+            self.Vtrap = trappot
+        self.rng = np.random.Generator(np.random.PCG64(seed))          # replaces GPEtask.m:49-52
+        self.history = {"mu": np.zeros(0), "n": np.zeros(0)}           # GPEtask.m:54
+        self.kinop = None
+        self.nlinop = None
+
+    # ------------------------------------------------------------------ helpers
+    def _check_omega(self):
+        if not np.isscalar(self.omega) or self.omega != 0:
+            raise NotImplementedError("rotating frame (omega != 0) is outside the restated path")
+
+    def getVtotal(self, time):
+        # GPEtask.m:62-71
+        if time == 0 and np.size(self.V0) > 1:
+            return self.V0
+        if callable(self.Vtd):
+            return self.Vtrap + self.Vtd(self, time)
+        return self.Vtrap
+
+    def applyham(self, phi, time=None):
+        # GPEtask.m:73-78
+        if time is None:
+            time = self.current_time
+        return self.applyh0(phi, time) + self.g * np.abs(phi) ** 2 * phi
+
+    def applyh0(self, phi=None, time=None):
+        # GPEtask.m:80-95 (omega == 0 branch: V.*phi + lap(phi))
+        self._check_omega()
+        if phi is None:
+            phi = self.current_state
+        if time is None:
+            time = self.current_time
+        res = self.getVtotal(time) * phi
+        res = res + self.grid.lap(phi)
+        return res
+
+    def get_energy(self, phi=None, time=None):
+        # GPEtask.m:97-107 (g halved for the interaction energy)
+        if phi is None:
+            phi = self.current_state
+        if time is None:
+            time = self.current_time
+        tmp = self.g
+        self.g = 0.5 * self.g
+        try:
+            res = np.real(self.grid.inner(phi, self.applyham(phi, time)))
+        finally:
+            self.g = tmp
+        return res
+
+    def set_kinop(self, dt):
+        # set_kinop.m:8-10 (omega == 0): kinop = exp(-kk*dt)
+        self._check_omega()
+        self.kinop = np.exp(-self.grid.kk * dt)
+
+    def ssft_kin_step(self, phi, dt):
+        # ssft_kin_step.m:17: phi = ifft(kinop .* fft(phi))
+        self._check_omega()
+        return self.grid.ifft(self.kinop * self.grid.fft(phi))
+
+    @staticmethod
+    def _hist_set(arr, step, val):
+        # MATLAB `history.mu(step) = val` grows the row with zeros (GPEtask.m:123,125)
+        if step > arr.size:
+            arr = np.concatenate([arr, np.zeros(step - arr.size, dtype=np.result_type(arr, val))])
+        elif np.iscomplexobj(val) and not np.iscomplexobj(arr):
+            arr = arr.astype(np.complex128)
+        arr[step - 1] = val
+        return arr
+
+    def ext_callback(self, phi, step, time, mu, n):
+        # GPEtask.m:115-142 (no snapshot dir, plot or console line)
+        self.current_state = phi
+        self.current_time = time
+        self.current_iter = step
+        self.current_mu = mu
+        self.history["mu"] = self._hist_set(self.history["mu"], step, mu)
+        self.current_n = n
+        self.history["n"] = self._hist_set(self.history["n"], step, n)
+        res_text = ""
+        if callable(self.user_callback):
+            r = self.user_callback(self)
+            res_text = r if isinstance(r, str) else ""
+        return res_text
+
+    def process_init_state(self, phi0):
+        # GPEtask.m:144-164
+        if isinstance(phi0, str):
+            if self.Ntotal > 0:
+                if phi0 == "tf":
+                    res, _ = self.groundstate_tf(_matlab_eps())
+                else:
+                    r = self.rng.random(self.grid.size) + 1j * self.rng.random(self.grid.size)
+                    res = np.sqrt(self.Ntotal) * self.grid.normalize(r)
+            else:
+                V = self.getVtotal(0)
+                res = np.real(np.sqrt((self.mu_init - V + 0j) / self.g))
+            return res
+        if self.Ntotal > 0:
+            return np.sqrt(self.Ntotal) * self.grid.normalize(phi0)
+        return phi0
+
+    # ------------------------------------------------------------------ solvers
+    def groundstate_tf(self, eps, N=None, V=None, mul=None, mur=None):
+        # groundstate_tf.m:1-37 (bisection on mu so that integrate((mu-V)_+) = N g)
+        if mul is None or mur is None:
+            if V is None:
+                if N is None:
+                    N = self.Ntotal
+                V = self.getVtotal(0)
+            mul = np.min(V)
+            mur = np.max(V)
+        g1 = np.atleast_1d(np.asarray(self.g, dtype=float)).ravel()[0]
+        NC = N * g1
+        mu = (mul + mur) / 2
+        vvv = None
+        while (mur - mul) / mu > eps:
+            vvv = (mu - V) * (V < mu)
+            NN = self.grid.integrate(vvv)
+            if NN > NC:
+                mur = mu
+            else:
+                mul = mu
+            mu = (mul + mur) / 2
+        phi = np.sqrt(vvv / g1)
+        return phi, mu
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """solve_split.m:1-84 -- Strang split-step with per-block open/close phases."""
+        grid = self.grid
+        g = self.g
+        if self.Ntotal > 0:                                      # 17-21
+            NN0 = self.Ntotal
+        else:
+            NN0 = grid.integrate(np.abs(self.init_state) ** 2)
+        NNN = NN0
+        gam = self.gamma
+        n_rec = self.n_recalc                                    # 24-27
+        if niter_inner % n_rec != 0:
+            n_rec = 1
+        tau = self.decay_rate
+        start = self.current_iter
+
+        dt = ddt * 1j / (1 + 1j * gam)                           # 31
+        self.set_kinop(dt)                                       # 32
+
+        phi = self.current_state if start > 0 else self.init_state   # 34-38
+
+        self.nlinop = np.real(phi * np.conj(phi))                # 40
+        muc = np.real(grid.inner(phi, self.applyham(phi))) / NN0     # 41
+        mu = self.mu_init if self.mu_init > 0 else muc           # 42-46
+        dt_outer = ddt * niter_inner
+        time2 = 0.0
+        ncur = NN0
+        for j in range(start + 1, niter_outer + 1):              # 49
+            time = (j - 1) * dt_outer
+            for jj in range(1, niter_inner // n_rec + 1):        # 51
+                time2 = time + (jj - 1) * ddt * n_rec            # 53
+                VV = self.getVtotal(time2) - mu                  # 54
+                self.nlinop = np.exp((-VV - g * self.nlinop) * dt * 0.5)   # 55
+                phi = self.nlinop * phi                          # 56
+                for _ in range(n_rec):                           # 58-62
+                    phi = self.ssft_kin_step(phi, dt)
+                    self.nlinop = np.exp((-VV - g * phi * np.conj(phi)) * dt)
+                    phi = self.nlinop * phi
+                self.nlinop = np.exp((VV + g * phi * np.conj(phi)) * dt * 0.5)   # 63
+                phi = self.nlinop * phi                          # 64
+                self.nlinop = np.real(phi * np.conj(phi))        # 65
+
+                ncur = grid.integrate(self.nlinop)               # 67
+                if gam > 0 and self.mu_init == 0:                # 68-78
+                    if tau > 0:
+                        NNN = NN0 * np.exp(-time2 / tau)
+                    phi = phi * np.sqrt(NNN / ncur)
+                    self.nlinop = self.nlinop * (NNN / ncur)
+                    muc = np.real(grid.inner(phi, self.applyham(phi, time2))) / NNN
+                    mu = muc
+                else:
+                    muc = np.real(grid.inner(phi, self.applyham(phi, time2))) / ncur
+            self.ext_callback(phi, j, time2, muc, ncur)          # 80
+        return phi
+
+    def groundstate_itp(self, dt, eps, phi0="rand", nargout=1):
+        """groundstate_itp.m:1-112 -- imaginary-time split step with renormalisation.
+
+        ``nargout`` plays MATLAB's nargout: 1 -> phi, 2 -> (phi, MU), 3 -> (+MU2),
+        4 -> (+EE).  MU2/EE are only evaluated when requested, as in the reference.
+        """
+        grid = self.grid
+        V = self.getVtotal(0)                                    # 20
+        nnn = self.Ntotal if self.Ntotal > 0 else 1              # 21-25
+        phi = self.process_init_state(phi0)                      # 27
+        self.current_state = phi                                 # 28
+        self.set_kinop(dt)                                       # 30
+        MU = np.zeros(self.itp_max_iter)                         # 31-33
+        MU2 = np.zeros(self.itp_max_iter)
+        EE = np.zeros(self.itp_max_iter)
+        i = 0
+        dtmin = np.sqrt(eps)
+        self.nlinop = np.exp(-dt * 0.5 * (np.abs(phi) ** 2 * self.g + V))   # 36
+        while True:
+            i += 1
+            phi = self.nlinop * phi                              # 40
+            phi = self.ssft_kin_step(phi, dt)                    # 41
+            phi = self.nlinop * phi                              # 42
+
+            self.nlinop = np.abs(phi) ** 2                       # 44
+            if self.Ntotal > 0:                                  # 45-51
+                mu = np.sqrt(self.Ntotal / grid.integrate(self.nlinop))
+                MU[i - 1] = np.log(mu) / dt
+            else:
+                mu = np.exp(self.mu_init * dt)
+                MU[i - 1] = grid.integrate(self.nlinop) * mu ** 2
+            phi = phi * mu                                       # 52
+
+            if i > self.itp_min_iter and i % 10 == 0:            # 54
+                if nargout >= 3:
+                    MU2[i - 1] = np.real(grid.inner(phi, self.applyham(phi)))
+                if nargout >= 4:
+                    EE[i - 1] = self.get_energy(phi)
+                delta = np.max(np.abs(np.abs(phi) - np.abs(self.current_state))) / (dt * 10)   # 61
+                self.current_state = phi
+                if delta < eps:
+                    if nargout >= 3:
+                        dtmin = dt * np.sqrt(eps / abs(MU[i - 1] - MU2[i - 1] / nnn))
+                    if (not self.itp_adjust_stepsize) or dt < dtmin:
+                        break
+                    dt = dt / 2
+                    self.set_kinop(dt)
+
+            self.nlinop = np.exp(-dt * 0.5 * (self.nlinop * self.g * mu ** 2 + V))   # 76
+
+            if i >= self.itp_max_iter:                           # 78-81
+                import warnings
+                warnings.warn("Convergence not reached")
+                break
+
+        MU = MU[:i]
+        out = [phi]
+        if nargout >= 2:                                         # 84-91
+            if self.Ntotal > 0:
+                self.current_mu = MU[-1]
+                self.current_n = self.Ntotal
+            out.append(MU)
+        if nargout >= 3:                                         # 92-100
+            if self.Ntotal > 0:
+                MU2 = MU2[:i] / self.Ntotal
+            else:
+                MU2 = MU2[:i] / MU
+                self.current_mu = MU2[-1]
+                self.current_n = MU[-1]
+            out.append(MU2)
+        if nargout >= 4:                                         # 101-108
+            EE = EE[:i] / self.Ntotal if self.Ntotal > 0 else EE[:i] / MU
+            out.append(EE)
+        self.init_state = phi                                    # 111
+        self.itp_iterations = i
+        return out[0] if nargout == 1 else tuple(out)
+
+
+class GPEtaskSpinor(GPEtask):
+    """GPEtaskSpinor.m:1-146 -- two coherently coupled components (omega = 0)."""
+
+    def __init__(self, grid, trappot, seed=0):
+        # GPEtaskSpinor.m:12-22
+        super().__init__(grid, trappot, seed)
+        self.coupling = 0.0
+        self.nlincpl = 0.0
+        self.ncomp = 0
+        if isinstance(trappot, (list, tuple)):
+            self.ncomp = len(trappot)
+            self.Vtrap = [t(grid.mesh.x, grid.mesh.y, grid.mesh.z) if callable(t) else t
+                          for t in trappot]
+
+    def getVtotal(self, time):
+        # GPEtaskSpinor.m:24-44
+        if self.Vtd is None:
+            return self.Vtrap
+        if time == 0 and np.size(self.V0) > 1:
+            return self.V0
+        if callable(self.Vtd):
+            return [self.Vtrap[i] + self.Vtd(self, time) for i in range(self.ncomp)]
+        if isinstance(self.Vtd, (list, tuple)):
+            return [self.Vtrap[i] + self.Vtd[i](self, time) for i in range(self.ncomp)]
+        return self.Vtrap
+
+    def applyham1c(self, phi, j, time=None):
+        # GPEtaskSpinor.m:55-75 (j is 0-based here)
+        self._check_omega()
+        if time is None:
+            time = self.current_time
+        res = self.getVtotal(time)[j]
+        g = np.asarray(self.g)
+        for k in range(self.ncomp):
+            res = res + g[j, k] * np.abs(phi[k]) ** 2
+        res = res * phi[j] + self.grid.lap(phi[j])
+        if j == 0:
+            res = res + self.coupling * phi[1]
+        else:
+            res = res + np.conj(self.coupling) * phi[0]
+        return res
+
+    def applyham(self, phi, time=None):
+        # GPEtaskSpinor.m:77-85
+        if time is None:
+            time = self.current_time
+        res = list(phi)
+        for j in range(self.ncomp):
+            res[j] = self.applyham1c(phi, j, time)
+        return res
+
+    def inner(self, left, right):
+        # GPEtaskSpinor.m:87-92
+        res = 0
+        for j in range(self.ncomp):
+            res = res + self.grid.inner(left[j], right[j])
+        return res
+
+    def get_energy(self, phi=None, time=None):
+        # GPEtaskSpinor.m:100-111
+        if time is None:
+            time = self.current_time
+        if phi is None:
+            phi = self.current_state
+        tmp = self.g
+        self.g = 0.5 * np.asarray(self.g)
+        try:
+            res = np.real(self.inner(phi, self.applyham(phi, time)))
+        finally:
+            self.g = tmp
+        return res
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """@GPEtaskSpinor/solve_split.m:1-121 -- K/2 [(P R) K]^n K^-1/2 per block."""
+        self._check_omega()
+        ncomp = np.asarray(self.g).shape[0]                      # 15
+        grid = self.grid
+        g = np.asarray(self.g)
+        gam = self.gamma
+        n_rec = self.n_recalc
+        tau = self.decay_rate
+        start = self.current_iter
+        phi = list(self.current_state) if start > 0 else list(self.init_state)   # 24-28
+        if self.Ntotal > 0:                                      # 30-38
+            NN0 = self.Ntotal
+        else:
+            raise ValueError("GPEtaskSpinor.solve_split needs Ntotal > 0 "
+                             "(solve_split.m:37 fails on a cell init_state otherwise)")
+        NNN = NN0
+        ncur = NN0
+        dt = ddt * 1j / (1 + 1j * gam)                           # 42
+        ekk = np.exp(-grid.kk * dt)                              # 43
+        ekkp = np.exp(-grid.kk * dt * 0.5)                       # 44
+        ekkm = np.exp(grid.kk * dt * 0.5)                        # 45
+        tmp2 = [None] * ncomp
+
+        hphi = self.applyham(phi)                                # 52
+        mu = 0
+        for j in range(ncomp):                                   # 53-56
+            mu = mu + np.real(grid.inner(phi[j], hphi[j]))
+        mu = mu / NN0                                            # 57
+        dt_outer = ddt * niter_inner
+        time2 = 0.0
+        for j in range(start + 1, niter_outer + 1):              # 60
+            coupl = self.coupling                                # 61-65
+            cang = np.angle(coupl)
+            cosom = np.cosh(dt * np.abs(coupl))
+            sinomm = np.sinh(dt * np.abs(coupl)) * np.exp(-1j * cang)
+            sinomp = np.sinh(dt * np.abs(coupl)) * np.exp(1j * cang)
+            time = (j - 1) * dt_outer
+            # 67: 1:niter_inner/n_rec truncates toward the floor in MATLAB's colon
+            for jj in range(1, int(np.floor(niter_inner / n_rec)) + 1):
+                time2 = time + (jj - 1) * ddt * n_rec            # 69
+                VV = self.getVtotal(time2)                       # 70
+                for i in range(ncomp):                           # 73-75
+                    phi[i] = grid.ifft(ekkp * grid.fft(phi[i]))
+                for _ in range(n_rec):                           # 76-96
+                    for i in range(ncomp):
+                        tmp2[i] = VV[i] - mu
+                        for k in range(ncomp):
+                            tmp2[i] = tmp2[i] + g[i, k] * np.abs(phi[k]) ** 2
+                    if self.nlincpl != 0:                        # 84-89
+                        tmp3 = coupl + self.nlincpl * (np.abs(phi[0]) ** 2 + np.abs(phi[1]) ** 2)
+                        cosom = np.cosh(dt * tmp3)
+                        sinomm = np.sinh(dt * tmp3)
+                        sinomp = sinomm
+                    t0 = np.exp(-tmp2[0] * dt) * (cosom * phi[0] - sinomp * phi[1])   # 90
+                    t1 = np.exp(-tmp2[1] * dt) * (cosom * phi[1] - sinomm * phi[0])   # 91
+                    tmp2[0], tmp2[1] = t0, t1
+                    for i in range(ncomp):                       # 92-94
+                        phi[i] = grid.ifft(ekk * grid.fft(tmp2[i]))
+                for i in range(ncomp):                           # 97-99
+                    phi[i] = grid.ifft(ekkm * grid.fft(phi[i]))
+                if gam > 0:                                      # 100-115
+                    if tau > 0:
+                        NNN = NN0 * np.exp(-time2 / tau)
+                    ncur = np.real(grid.integrate(phi[0] * np.conj(phi[0]) + phi[1] * np.conj(phi[1])))
+                    for i in range(ncomp):
+                        phi[i] = phi[i] * np.sqrt(NNN / ncur)
+                    hphi = self.applyham(phi)
+                    mu = self.inner(phi, hphi) / NNN
+                else:
+                    ncur = np.real(grid.integrate(phi[0] * np.conj(phi[0]) + phi[1] * np.conj(phi[1])))
+                    hphi = self.applyham(phi)
+                    mu = self.inner(phi, hphi) / NNN
+            self.ext_callback(list(phi), j, time2, mu, ncur)     # 117
+        return phi

@@ -1,0 +1,73 @@
+"""CPU checks of the C-ABI library: it loads, exports every declared entry point,
+and rejects bad arguments without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "gpe_hip.h")
+LIB = os.path.join(ROOT, "gpe-matlab_amd", "gpe_hip", "libgpehip.so")
+
+
+def declared_symbols():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpe_[a-z_0-9]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} missing: run `python gpe-matlab_amd/build.py` (or __graft_entry__.build())")
+    return C.CDLL(LIB)
+
+
+def test_header_declares_core_entry_points():
+    syms = declared_symbols()
+    for s in ["gpe_create", "gpe_destroy", "gpe_set_params", "gpe_set_potential", "gpe_set_state",
+              "gpe_get_state", "gpe_split_begin", "gpe_split_outer", "gpe_split_block", "gpe_itp",
+              "gpe_measure", "gpe_last_error"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_ctypes_signature_table_matches_header():
+    import gpe_hip._lib as L
+    syms = set(declared_symbols()) - {"gpe_last_error"}
+    assert syms == set(L.SIGNATURES), (syms ^ set(L.SIGNATURES))
+
+
+def test_create_rejects_bad_grids(lib):
+    import gpe_hip._lib as L
+    lib.gpe_create.argtypes = [C.POINTER(L.GridDesc), C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    d = L.GridDesc()
+    d.ndims = 4
+    assert lib.gpe_create(C.byref(d), C.byref(h)) == -1
+    d.ndims = 2
+    d.n[0], d.n[1] = 100, 64          # 100 has a factor 5: unsupported length
+    d.xmax[0], d.xmax[1] = 1.0, 1.0
+    assert lib.gpe_create(C.byref(d), C.byref(h)) == -1
+    d.n[0] = 64
+    d.xmax[1] = -1.0
+    assert lib.gpe_create(C.byref(d), C.byref(h)) == -1
+    lib.gpe_last_error.restype = C.c_char_p
+    lib.gpe_last_error.argtypes = [C.c_void_p]
+    assert lib.gpe_last_error(None) == b"null handle"
+
+
+def test_package_fails_loudly_without_library(monkeypatch, tmp_path):
+    import importlib
+    import gpe_hip._lib as L
+    monkeypatch.setattr(L, "LIBPATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(L.GPELibraryError):
+        L.load()
+    importlib.reload(L)

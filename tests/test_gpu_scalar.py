@@ -1,0 +1,195 @@
+"""GPU parity of GPEtask.solve_split / groundstate_itp against the FP64 oracle.
+
+Every test runs the HIP path through the C ABI (gpe_hip -> libgpehip.so) and the
+oracle restatement on the same seeded inputs.  Tolerance: relative L2 <= 1e-11 on
+the final state, scalars to 1e-11 relative unless stated.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def _pair(dims, n, xmax):
+    import gpe_hip
+    if dims == 2:
+        return gpe_hip.grid2d(xmax[0], n[0], xmax[1], n[1]), oracle.grid2d(xmax[0], n[0], xmax[1], n[1])
+    return (gpe_hip.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2]),
+            oracle.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2]))
+
+
+def _state(grid, seed, vortex=True):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r2 = grid.mesh.x ** 2 + grid.mesh.y ** 2 + (grid.mesh.z ** 2 if grid.ndims == 3 else 0)
+    phi = np.exp(-r2 / 4) * np.exp(1j * rng.uniform(0, 0.1, grid.size))
+    if vortex:
+        phi = phi * np.exp(1j * np.arctan2(grid.mesh.x - 0.5, grid.mesh.y))
+    return phi
+
+
+def _tasks(dims, n, xmax, V, g=0.5, gamma=0.0, Ntotal=10.0, seed=3, **props):
+    import gpe_hip
+    G, O = _pair(dims, n, xmax)
+    tg, to = gpe_hip.GPEtask(G, V), oracle.GPEtask(O, V)
+    phi = _state(G, seed)
+    for t in (tg, to):
+        t.g, t.gamma, t.Ntotal = g, gamma, Ntotal
+        for k, v in props.items():
+            setattr(t, k, v)
+        t.init_state = phi.copy()
+    return tg, to
+
+
+def harm2(X, Y, Z):
+    return 0.5 * (X ** 2 + 2.0 * Y ** 2)
+
+
+def harm3(X, Y, Z):
+    return 0.5 * (X ** 2 + Y ** 2 + 4.0 * Z ** 2)
+
+
+CASES_2D = [((16, 24), (6, 5)), ((32, 32), (8, 8)), ((64, 48), (8, 7)), ((12, 96), (5, 9)),
+            ((128, 8), (9, 4)), ((256, 192), (10, 9)), ((384, 16), (12, 4)), ((512, 32), (14, 5))]
+
+
+@pytest.mark.parametrize("n,xmax", CASES_2D)
+@pytest.mark.parametrize("gamma", [0.0, 0.03])
+def test_solve_split_2d_parity(n, xmax, gamma):
+    tg, to = _tasks(2, n, xmax, harm2, gamma=gamma)
+    pg = tg.solve_split(0.004, 20, 3)
+    po = to.solve_split(0.004, 20, 3)
+    err = rel_l2(pg, po)
+    print(f"2D {n} gamma={gamma}: rel-L2 vs oracle = {err:.3e}")
+    assert err < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=TOL, atol=0)
+    assert tg.current_time == to.current_time and tg.current_iter == to.current_iter
+
+
+CASES_3D = [((16, 16, 16), (5, 5, 5)), ((24, 16, 32), (6, 5, 7)), ((32, 48, 12), (7, 8, 4)),
+            ((64, 64, 64), (8, 8, 8)), ((48, 96, 32), (7, 9, 6))]
+
+
+@pytest.mark.parametrize("n,xmax", CASES_3D)
+@pytest.mark.parametrize("gamma", [0.0, 0.02])
+def test_solve_split_3d_parity(n, xmax, gamma):
+    tg, to = _tasks(3, n, xmax, harm3, gamma=gamma)
+    pg = tg.solve_split(0.002, 20, 2)
+    po = to.solve_split(0.002, 20, 2)
+    err = rel_l2(pg, po)
+    print(f"3D {n} gamma={gamma}: rel-L2 vs oracle = {err:.3e}")
+    assert err < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=TOL, atol=0)
+
+
+def test_solve_split_options_parity():
+    """n_rec fallback, decay_rate, mu_init, and resume across calls (example2 pattern)."""
+    for props, ni in [({"decay_rate": 0.5}, 20), ({"mu_init": 1.7}, 20), ({}, 7), ({"n_recalc": 5}, 15)]:
+        tg, to = _tasks(2, (32, 24), (7, 6), harm2, gamma=0.05, **props)
+        for no in (1, 2, 3):                       # solve_split(tstep, steps_int, i)
+            pg = tg.solve_split(0.004, ni, no)
+            po = to.solve_split(0.004, ni, no)
+        err = rel_l2(pg, po)
+        print(f"options {props} ni={ni}: rel-L2 = {err:.3e}")
+        assert err < TOL
+        assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+
+
+def test_solve_split_time_dependent_potential_parity():
+    """Vtd evaluated per block at time2 (GPEtask.m:65-67, solve_split.m:54)."""
+    def vtd(task, t):
+        return 0.3 * np.sin(2.0 * t) * task.grid.mesh.x
+
+    tg, to = _tasks(2, (32, 32), (8, 8), harm2, gamma=0.01)
+    tg.Vtd = vtd
+    to.Vtd = vtd
+    pg = tg.solve_split(0.004, 20, 3)
+    po = to.solve_split(0.004, 20, 3)
+    assert rel_l2(pg, po) < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+
+
+def test_linear_closed_form_on_gpu():
+    """g=0, V=c: two plane waves acquire e^{-i(E_j-mu)t}; pins the whole step independently of the oracle."""
+    import gpe_hip
+    grid = gpe_hip.grid2d(6, 16, 5, 12)
+    c = 0.7
+    t = gpe_hip.GPEtask(grid, c + 0 * grid.mesh.x)
+    kx1 = 2 * np.pi / (grid.x[-1] - grid.x[0] + grid.dx)
+    ky1 = 2 * np.pi / (grid.y[-1] - grid.y[0] + grid.dy)
+    k1, k2 = (2 * kx1, 1 * ky1), (-3 * kx1, 3 * ky1)
+    w1 = np.exp(1j * (k1[0] * grid.mesh.x + k1[1] * grid.mesh.y))
+    w2 = np.exp(1j * (k2[0] * grid.mesh.x + k2[1] * grid.mesh.y))
+    phi = 0.8 * w1 + 0.6 * w2
+    t.Ntotal = grid.integrate(np.abs(phi) ** 2)
+    t.init_state = phi
+    out = t.solve_split(0.01, 20, 3)
+    E1 = (k1[0] ** 2 + k1[1] ** 2) / 2 + c
+    E2 = (k2[0] ** 2 + k2[1] ** 2) / 2 + c
+    mu = 0.64 * E1 + 0.36 * E2
+    T = 0.6
+    expect = 0.8 * np.exp(-1j * (E1 - mu) * T) * w1 + 0.6 * np.exp(-1j * (E2 - mu) * T) * w2
+    assert rel_l2(out, expect) < 1e-12
+    assert abs(t.history["mu"][-1] / mu - 1) < 1e-12
+
+
+@pytest.mark.parametrize("dims,n,xmax", [(2, (64, 64), (8, 8)), (3, (32, 32, 24), (6, 6, 5))])
+def test_itp_fixed_iterations_parity(dims, n, xmax):
+    """groundstate_itp at a fixed iteration count (itp_min_iter = itp_max_iter)."""
+    import gpe_hip
+    G, O = _pair(dims, n, xmax)
+    V = harm2 if dims == 2 else harm3
+    tg, to = gpe_hip.GPEtask(G, V, seed=5), oracle.GPEtask(O, V, seed=5)
+    for t in (tg, to):
+        t.g, t.Ntotal = 0.8, 50.0
+        t.itp_min_iter = t.itp_max_iter = 60
+    with pytest.warns(UserWarning):
+        pg, MUg = tg.groundstate_itp(0.01, 1e-12, "rand", nargout=2)
+    with pytest.warns(UserWarning):
+        po, MUo = to.groundstate_itp(0.01, 1e-12, "rand", nargout=2)
+    err = rel_l2(pg, po)
+    print(f"ITP {n}: rel-L2 = {err:.3e}")
+    assert err < TOL
+    assert np.allclose(MUg, MUo, rtol=1e-10, atol=1e-12)
+
+
+def test_itp_converged_parity():
+    """groundstate_itp to convergence with MU, MU2, EE histories (nargout = 4)."""
+    import gpe_hip
+    G, O = _pair(2, (64, 64), (8, 8))
+    tg, to = gpe_hip.GPEtask(G, harm2, seed=9), oracle.GPEtask(O, harm2, seed=9)
+    for t in (tg, to):
+        t.g, t.Ntotal = 0.5, 20.0
+    pg, MUg, MU2g, EEg = tg.groundstate_itp(0.01, 1e-6, "tf", nargout=4)
+    po, MUo, MU2o, EEo = to.groundstate_itp(0.01, 1e-6, "tf", nargout=4)
+    assert tg.itp_iterations == to.itp_iterations
+    assert rel_l2(pg, po) < TOL
+    assert np.allclose(MUg, MUo, rtol=1e-10, atol=1e-12)
+    assert np.allclose(MU2g, MU2o, rtol=1e-10, atol=1e-12)
+    assert np.allclose(EEg, EEo, rtol=1e-10, atol=1e-12)
+
+
+def test_energy_measure_parity():
+    import gpe_hip
+    G, O = _pair(3, (32, 24, 16), (6, 5, 4))
+    phi = _state(G, 11)
+    tg, to = gpe_hip.GPEtask(G, harm3), oracle.GPEtask(O, harm3)
+    tg.g = to.g = 0.7
+    eg = tg.get_energy(phi)
+    eo = to.get_energy(phi)
+    assert abs(eg / eo - 1) < 1e-12
+
+
+def test_determinism_bitwise():
+    """Two identical runs give bit-identical states (fixed-order reductions)."""
+    outs = []
+    for _ in range(2):
+        tg, _o = _tasks(3, (32, 32, 32), (6, 6, 6), harm3, gamma=0.01)
+        outs.append(tg.solve_split(0.002, 20, 2))
+    assert np.array_equal(outs[0], outs[1])

