@@ -133,6 +133,11 @@ int gpe_destroy(gpe_handle* h) {
   hipFree(h->partials);
   hipFree(h->st);
   hipFree(h->MUdev);
+  for (auto& r : h->prof_recs) {
+    hipEventDestroy(r.second.first);
+    hipEventDestroy(r.second.second);
+  }
+  for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return GPE_OK;
@@ -397,6 +402,33 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
   if (dt_final) *dt_final = dt;
   if (converged) *converged = conv;
   h->begun = false;
+  return GPE_OK;
+}
+
+int gpe_profile_enable(gpe_handle* h, int enable) {
+  if (!h) return GPE_EINVAL;
+  h->prof = enable != 0;
+  return GPE_OK;
+}
+
+int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n) {
+  if (!h || !ms || !count || n < gpe::S_NSEQ * 3) return h ? fail(h, GPE_EINVAL, "profile buffer too small") : GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n; ++i) {
+    ms[i] = 0.0;
+    count[i] = 0;
+  }
+  for (auto& r : h->prof_recs) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.second.first, r.second.second) == hipSuccess) {
+      ms[r.first] += t;
+      count[r.first] += 1;
+    }
+    h->ev_pool.push_back(r.second.first);
+    h->ev_pool.push_back(r.second.second);
+  }
+  h->prof_recs.clear();
   return GPE_OK;
 }
 

@@ -65,6 +65,10 @@ struct gpe_handle {
   int niter_inner = 0, n_rec = 1;
   double NN0 = 0;
   std::string err;
+  // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_recs;
 };
 
 namespace gpe_detail {
@@ -165,9 +169,30 @@ inline SweepParams base_params(gpe_handle* h, int axis, double2* psi, const doub
   return p;
 }
 
+inline hipEvent_t prof_event(gpe_handle* h) {
+  hipEvent_t e = nullptr;
+  if (!h->ev_pool.empty()) {
+    e = h->ev_pool.back();
+    h->ev_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
 inline int sweep(gpe_handle* h, int axis, int seq, const SweepParams& p, int* nb = nullptr) {
   const bool colfast = axis != 1;  // y is contiguous: threads run along the pencil
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof) {
+    e0 = prof_event(h);
+    e1 = prof_event(h);
+    if (e0) hipEventRecord(e0, h->stream);
+  }
   hipError_t e = gpe::launch_sweep(h->ax[axis].n, seq, colfast, p, h->stream, nb);
+  if (h->prof && e0 && e1) {
+    hipEventRecord(e1, h->stream);
+    h->prof_recs.push_back({seq * 3 + axis, {e0, e1}});
+  }
   if (e != hipSuccess) return fail(h, GPE_EHIP, "sweep launch (axis %d, seq %d): %s", axis, seq, hipGetErrorString(e));
   return GPE_OK;
 }

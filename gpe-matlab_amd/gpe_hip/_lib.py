@@ -52,6 +52,8 @@ SIGNATURES = {
     "gpe_measure": [_P, _DP],
     "gpe_itp": [_P, C.POINTER(ItpArgs), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_device_info": [C.c_char_p, C.c_int],
+    "gpe_profile_enable": [_P, C.c_int],
+    "gpe_profile_read": [_P, _DP, _IP, C.c_int],
 }
 OPTIONAL = set()
 
@@ -209,6 +211,20 @@ class Device:
         self.call("gpe_itp", C.byref(a), dptr(MU), dptr(MU2), dptr(EE), C.byref(nit), C.byref(dtf), C.byref(conv))
         i = nit.value
         return MU[:i], MU2[:i], EE[:i], i, dtf.value, bool(conv.value)
+
+    def profile_enable(self, on=True):
+        self.call("gpe_profile_enable", int(bool(on)))
+
+    def profile_read(self):
+        """{(seq, axis): (total_ms, launches)} of the sweeps since the last read."""
+        ms = np.zeros(30)
+        cnt = np.zeros(30, dtype=np.int32)
+        self.call("gpe_profile_read", dptr(ms), cnt.ctypes.data_as(_IP), 30)
+        return {(i // 3, i % 3): (float(ms[i]), int(cnt[i])) for i in range(30) if cnt[i] > 0}
+
+
+SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb"}
+AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
 
 
 def device_info():

@@ -112,6 +112,13 @@ typedef struct {
 int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, double* EE, int* n_iter,
             double* dt_final, int* converged);
 
+/* Measurement hooks (no reference counterpart): time every sweep launch with
+ * HIP events recorded on the handle's own stream.  gpe_profile_read fills
+ * ms[seq*3 + axis] (summed milliseconds) and count[seq*3 + axis] for
+ * seq < 10 (the sweep sequences) and axis 0..2 (x, y, z); n >= 30. */
+int gpe_profile_enable(gpe_handle* h, int enable);
+int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n);
+
 /* Library/device information for diagnostics: writes a NUL-terminated string. */
 int gpe_device_info(char* buf, int len);
 
