@@ -48,6 +48,10 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   h->ny = d->n[1];
   h->nz = d->ndims == 3 ? d->n[2] : 1;
   h->npts = (long long)h->nx * h->ny * h->nz;
+  if (h->npts > (1LL << 27)) {  // kernels index the grid with 32-bit offsets
+    delete h;
+    return GPE_EINVAL;
+  }
   const int nn[3] = {h->nx, h->ny, h->nz};
   double dd[3] = {0, 0, 1};
   for (int a = 0; a < h->ndims; ++a) {
@@ -430,6 +434,24 @@ int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n) {
   }
   h->prof_recs.clear();
   return GPE_OK;
+}
+
+int gpe_selftest_phase(int n, const double* er, const double* ei, double* out) {
+  if (n <= 0 || !er || !ei || !out) return GPE_EINVAL;
+  double *der = nullptr, *dei = nullptr;
+  double2* dout = nullptr;
+  int rc = GPE_EHIP;
+  if (hipMalloc(&der, n * sizeof(double)) == hipSuccess && hipMalloc(&dei, n * sizeof(double)) == hipSuccess &&
+      hipMalloc(&dout, n * sizeof(double2)) == hipSuccess &&
+      hipMemcpy(der, er, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dei, ei, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+      gpe::launch_phase_selftest(der, dei, dout, n, nullptr) == hipSuccess &&
+      hipMemcpy(out, dout, n * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess)
+    rc = GPE_OK;
+  hipFree(der);
+  hipFree(dei);
+  hipFree(dout);
+  return rc;
 }
 
 }  // extern "C"

@@ -156,11 +156,11 @@ inline SweepParams base_params(gpe_handle* h, int axis, double2* psi, const doub
   p.nop = h->nop;
   p.partials = h->partials;
   p.st = h->st;
-  p.ncols = a.ncols;
-  p.A = a.A;
-  p.sa = a.sa;
-  p.sb = a.sb;
-  p.stride = a.stride;
+  p.ncols = (unsigned)a.ncols;
+  p.A = (unsigned)a.A;
+  p.sa = (unsigned)a.sa;
+  p.sb = (unsigned)a.sb;
+  p.stride = (unsigned)a.stride;
   p.maxwg = h->maxwg;
   p.slot_ke = gpe::SL_KX + axis;
   p.g = h->prm.g[0];

@@ -27,5 +27,6 @@ hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st
                             int maxwg, int nblocks, hipStream_t s);
 hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s);
 int sweep_cols_per_block(int n);
+hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
 
 }  // namespace gpe

@@ -1,6 +1,7 @@
 // Block-boundary kernels: second-stage reductions, device-side scalar updates
 // (norm, renormalisation, chemical potential) and small elementwise passes.
 #include "fft_dev.h"
+#include "phase_dev.h"
 #include "reduce_dev.h"
 #include "state.h"
 
@@ -100,6 +101,17 @@ __global__ void reset_scale_kernel(DevState* st, int which) {
   if (threadIdx.x == 0) {
     if (which == 0) st->scale = 1.0; else st->mu_itp = 1.0;
   }
+}
+
+// out[i] = (exp(er) cos(ei), exp(er) sin(ei)) through the sweep kernels' phase code.
+__global__ void phase_selftest_kernel(const double* er, const double* ei, double2* out, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = mul_cexp(make_double2(1.0, 0.0), er[i], ei[i], true);
+}
+
+hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s) {
+  hipLaunchKernelGGL(phase_selftest_kernel, dim3((n + 255) / 256), dim3(256), 0, s, er, ei, out, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_finalize(const FinParams& f, hipStream_t s) {

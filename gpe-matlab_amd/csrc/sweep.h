@@ -51,9 +51,9 @@ struct SweepParams {
   double* nop;               // stored real ITP operator (S_ITPA reads, S_ITPB writes)
   double* partials;          // reduction partials [slot * maxwg + block]
   DevState* st;              // device scalars
-  long long ncols;           // number of pencils
-  long long A, sa, sb;       // pencil base = (col % A) * sa + (col / A) * sb
-  long long stride;          // element stride along the pencil
+  unsigned ncols;            // number of pencils
+  unsigned A, sa, sb;        // pencil base = (col % A) * sa + (col / A) * sb
+  unsigned stride;           // element stride along the pencil (grid <= 2^27 points)
   int maxwg;                 // slot pitch of partials
   int slot_ke;               // slot receiving k^2 sums
   double g;                  // interaction constant

@@ -54,6 +54,7 @@ SIGNATURES = {
     "gpe_device_info": [C.c_char_p, C.c_int],
     "gpe_profile_enable": [_P, C.c_int],
     "gpe_profile_read": [_P, _DP, _IP, C.c_int],
+    "gpe_selftest_phase": [C.c_int, _DP, _DP, _DP],
 }
 OPTIONAL = set()
 
@@ -225,6 +226,17 @@ class Device:
 
 SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
+
+
+def selftest_phase(er, ei):
+    """Device phase exp(er) * (cos ei, sin ei) as complex128 (test hook)."""
+    lib = load()
+    er = np.ascontiguousarray(er, dtype=np.float64)
+    ei = np.ascontiguousarray(ei, dtype=np.float64)
+    out = np.empty(er.size, dtype=np.complex128)
+    check(lib.gpe_selftest_phase(er.size, dptr(er), dptr(ei), out.view(np.float64).ctypes.data_as(_DP)),
+          None, "gpe_selftest_phase")
+    return out
 
 
 def device_info():

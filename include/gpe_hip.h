@@ -119,6 +119,11 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
 int gpe_profile_enable(gpe_handle* h, int enable);
 int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n);
 
+/* Self-test hook (no reference counterpart): out[2i], out[2i+1] =
+ * exp(er[i]) * (cos(ei[i]), sin(ei[i])) computed by the device phase code the
+ * sweep kernels use, for checking it against a host math library. */
+int gpe_selftest_phase(int n, const double* er, const double* ei, double* out);
+
 /* Library/device information for diagnostics: writes a NUL-terminated string. */
 int gpe_device_info(char* buf, int len);
 

@@ -193,3 +193,21 @@ def test_determinism_bitwise():
         tg, _o = _tasks(3, (32, 32, 32), (6, 6, 6), harm3, gamma=0.01)
         outs.append(tg.solve_split(0.002, 20, 2))
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_device_phase_accuracy():
+    """Device exp/sincos (Cody-Waite + Taylor) against numpy: <= 2 ulp-scale error."""
+    from gpe_hip._lib import selftest_phase
+    rng = np.random.Generator(np.random.PCG64(17))
+    ei = np.concatenate([rng.uniform(-10, 10, 200000), rng.uniform(-1e5, 1e5, 50000),
+                         np.array([0.0, np.pi / 2, -np.pi, 1e-300, 5e-324, 3.0 * np.pi / 4])])
+    er = np.concatenate([rng.uniform(-0.1, 0.1, 100000), rng.uniform(-700, 700, 150000), np.zeros(6)])
+    out = selftest_phase(er, ei)
+    ref = np.exp(er) * (np.cos(ei) + 1j * np.sin(ei))
+    scale = np.exp(er)
+    err = np.abs(out - ref) / scale
+    print(f"phase max abs err / e^x: {err.max():.3e}")
+    assert err.max() < 1e-15
+    # special values propagate like the library functions
+    sp = selftest_phase(np.array([np.nan, 0.0, -1000.0, 1000.0]), np.array([0.0, np.inf, 1.0, 0.0]))
+    assert np.isnan(sp[0].real) and np.isnan(sp[1].real) and sp[2] == 0 and np.isinf(sp[3].real)
