@@ -130,8 +130,10 @@ int gpe_destroy(gpe_handle* h) {
   hipFree(h->cpl);
   hipFree(h->kcol);
   hipFree(h->kax);
-  hipFree(h->kcol2);
-  hipFree(h->kax2);
+  for (int i = 0; i < 2; ++i) {
+    hipFree(h->kcol_sp[i]);
+    hipFree(h->kax_sp[i]);
+  }
   hipFree(h->nop);
   hipFree(h->psi_old);
   hipFree(h->partials);

@@ -47,8 +47,9 @@ struct gpe_handle {
   bool cpl_array = false;
   double2* kcol = nullptr;      // kinetic factor per mid-axis column
   double2* kax = nullptr;       // kinetic factor per mid-axis index (incl. 1/npts)
-  double2* kcol2 = nullptr;     // spinor: second kinetic table pair
-  double2* kax2 = nullptr;
+  double2* kcol_sp[2] = {nullptr, nullptr};  // spinor: exp(-kk dt/2) and exp(-kk dt) exp(kk dt/2)
+  double2* kax_sp[2] = {nullptr, nullptr};
+  double cosom[2] = {1, 0}, sinomm[2] = {0, 0}, sinomp[2] = {0, 0};  // scalar Rabi coefficients
   double* nop = nullptr;        // ITP stored operator
   double2* psi_old = nullptr;   // ITP convergence reference
   double* partials = nullptr;
@@ -201,7 +202,8 @@ inline int mid_axis(const gpe_handle* h) { return h->ndims == 3 ? 2 : 1; }
 
 // Kinetic tables for complex dt: kcol[col] * kax[i] = exp(-kk dt) / npts
 // (set_kinop.m:9) split into the mid-spectrum axis and its columns.
-inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, double2* kax, double frac = 1.0) {
+inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, double2* kax, double frac = 1.0,
+                         double frac2 = 0.0) {
   const int ma = mid_axis(h);
   const Axis& m = h->ax[ma];
   std::vector<double2> col, axv(m.n);
@@ -210,21 +212,25 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
     for (int ix = 0; ix < h->nx; ++ix)
       for (int iy = 0; iy < h->ny; ++iy) {
         const double kk = (h->ax[0].k[ix] * h->ax[0].k[ix] + h->ax[1].k[iy] * h->ax[1].k[iy]) / 2;
-        const std::complex<double> e = std::exp(-kk * dt * frac);
+        std::complex<double> e = std::exp(-kk * dt * frac);
+        if (frac2 != 0.0) e *= std::exp(-kk * dt * frac2);
         col[(size_t)iy + (size_t)h->ny * ix] = make_double2(e.real(), e.imag());
       }
   } else {
     col.resize(h->nx);
     for (int ix = 0; ix < h->nx; ++ix) {
       const double kk = h->ax[0].k[ix] * h->ax[0].k[ix] / 2;
-      const std::complex<double> e = std::exp(-kk * dt * frac);
+      std::complex<double> e = std::exp(-kk * dt * frac);
+      if (frac2 != 0.0) e *= std::exp(-kk * dt * frac2);
       col[ix] = make_double2(e.real(), e.imag());
     }
   }
   const double inv = 1.0 / (double)h->npts;
   for (int i = 0; i < m.n; ++i) {
     const double kk = m.k[i] * m.k[i] / 2;
-    const std::complex<double> e = std::exp(-kk * dt * frac) * inv;
+    std::complex<double> e = std::exp(-kk * dt * frac);
+    if (frac2 != 0.0) e *= std::exp(-kk * dt * frac2);
+    e *= inv;
     axv[i] = make_double2(e.real(), e.imag());
   }
   HIPCHK(h, hipMemcpyAsync(kcol, col.data(), col.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
@@ -317,16 +323,9 @@ inline int materialize(gpe_handle* h) {
   int rc;
   if ((rc = read_state(h, &s))) return rc;
   if (s.scale != 1.0) {
-    for (int c = 0; c < h->prm.ncomp; ++c) {
-      HIPCHK(h, gpe::launch_scale(h->psi[c], h->st, 0, h->npts, elem_blocks(h->npts), h->stream));
-      if (c + 1 < h->prm.ncomp) {
-        // launch_scale resets the scale; restore it for the next component
-        DevState t;
-        if ((rc = read_state(h, &t))) return rc;
-        t.scale = s.scale;
-        if ((rc = write_state(h, t))) return rc;
-      }
-    }
+    for (int c = 0; c < h->prm.ncomp; ++c)
+      HIPCHK(h, gpe::launch_scale(h->psi[c], h->st, 0, h->npts, elem_blocks(h->npts), h->stream,
+                                  c + 1 == h->prm.ncomp));
     HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return GPE_OK;

@@ -4,7 +4,7 @@
 
 namespace gpe {
 
-enum FinMode : int { FIN_SUMS = 0, FIN_BLOCK = 1, FIN_ITP = 2, FIN_MAX = 3 };
+enum FinMode : int { FIN_SUMS = 0, FIN_BLOCK = 1, FIN_ITP = 2, FIN_MAX = 3, FIN_SPINOR = 4 };
 
 struct FinParams {
   const double* partials;
@@ -20,12 +20,14 @@ struct FinParams {
   double Ntotal, mu_init;
   double* MU;          // FIN_ITP: device history
   int iter;
+  double g11, g12, g21, g22;   // FIN_SPINOR: interaction matrix
 };
 
 hipError_t launch_finalize(const FinParams& f, hipStream_t s);
 hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st, long long n, double* partials,
                             int maxwg, int nblocks, hipStream_t s);
-hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s);
+hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s,
+                        bool reset = true);
 int sweep_cols_per_block(int n);
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
 

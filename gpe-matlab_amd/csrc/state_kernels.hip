@@ -34,9 +34,18 @@ __global__ void __launch_bounds__(256) finalize_kernel(const FinParams f) {
   const double kx = f.cnt[SL_KX] ? slot_sum(f.partials, f.maxwg, SL_KX, f.cnt[SL_KX], red) : 0.0;
   const double ky = f.cnt[SL_KY] ? slot_sum(f.partials, f.maxwg, SL_KY, f.cnt[SL_KY], red) : 0.0;
   const double kz = f.cnt[SL_KZ] ? slot_sum(f.partials, f.maxwg, SL_KZ, f.cnt[SL_KZ], red) : 0.0;
+  const double c12 = f.cnt[SL_C12] ? slot_sum(f.partials, f.maxwg, SL_C12, f.cnt[SL_C12], red) : 0.0;
+  const double c2 = f.cnt[SL_C2] ? slot_sum(f.partials, f.maxwg, SL_C2, f.cnt[SL_C2], red) : 0.0;
+  const double sx = f.cnt[SL_X] ? slot_sum(f.partials, f.maxwg, SL_X, f.cnt[SL_X], red) : 0.0;
+  const double kx2 = f.cnt[SL_KX2] ? slot_sum(f.partials, f.maxwg, SL_KX2, f.cnt[SL_KX2], red) : 0.0;
+  const double ky2 = f.cnt[SL_KY2] ? slot_sum(f.partials, f.maxwg, SL_KY2, f.cnt[SL_KY2], red) : 0.0;
+  const double kz2 = f.cnt[SL_KZ2] ? slot_sum(f.partials, f.maxwg, SL_KZ2, f.cnt[SL_KZ2], red) : 0.0;
   if (threadIdx.x != 0) return;
-  const double sK = (kx + ky) + kz;
-  st->sA = sA; st->sB = sB; st->sC = sC; st->sK = sK;
+  const double sK = ((kx + ky) + kz) + ((kx2 + ky2) + kz2);
+  st->sA = sA; st->sB = sB; st->sK = sK;
+  // scalar runs: sC = sum |psi|^4; spinor runs: sC = sum_jk g_jk n_j n_k, sX = sum Re conj(psi1) Omega psi2
+  st->sC = f.mode == FIN_SPINOR || f.cnt[SL_C12] ? (f.g11 * sC + (f.g12 + f.g21) * c12) + f.g22 * c2 : sC;
+  st->pad[0] = sx;
   const double w = f.w;
   if (f.mode == FIN_BLOCK) {
     // solve_split.m:65-78
@@ -57,6 +66,21 @@ __global__ void __launch_bounds__(256) finalize_kernel(const FinParams f) {
     }
     st->muc = muc;
     st->ncur = ncur;
+  } else if (f.mode == FIN_SPINOR) {
+    // lib/@GPEtaskSpinor/solve_split.m:100-115: ncur, optional renormalisation, mu = <phi|H phi>/NNN
+    const double ncur = sA * w;
+    double NNN = st->NNN;
+    double q = 1.0;
+    if (f.renorm) {
+      if (f.tau > 0) NNN = f.NN0 * exp(-f.time2 / f.tau);
+      q = NNN / ncur;
+    }
+    st->scale = f.renorm ? sqrt(q) : 1.0;
+    const double mu = (q * ((sB + sK) + 2.0 * sx) + (q * q) * st->sC) * w / NNN;
+    st->mu = mu;
+    st->muc = mu;
+    st->ncur = ncur;
+    st->NNN = NNN;
   } else if (f.mode == FIN_ITP) {
     // groundstate_itp.m:44-52
     double mu;
@@ -125,9 +149,10 @@ hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st
   return hipGetLastError();
 }
 
-hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s) {
+hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s,
+                        bool reset) {
   hipLaunchKernelGGL(scale_kernel, dim3(nblocks), dim3(256), 0, s, psi, st, which, n);
-  hipLaunchKernelGGL(reset_scale_kernel, dim3(1), dim3(64), 0, s, st, which);
+  if (reset) hipLaunchKernelGGL(reset_scale_kernel, dim3(1), dim3(64), 0, s, st, which);
   return hipGetLastError();
 }
 
@@ -163,6 +188,22 @@ int sweep_cols_per_block(int n) {
     case 384: return 256 / Plan<384>::T;
     case 512: return 256 / Plan<512>::T;
     default: return 0;
+  }
+}
+
+#define GPE_DECL(N) hipError_t launch_spinor_sweep_##N(int, const SpinorParams&, hipStream_t, int*);
+GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
+GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
+#undef GPE_DECL
+
+hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks) {
+  switch (n) {
+#define GPE_C(N) case N: return launch_spinor_sweep_##N(seq, p, s, nblocks);
+    GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
+    GPE_C(256) GPE_C(384) GPE_C(512)
+#undef GPE_C
+    default:
+      return hipErrorInvalidValue;
   }
 }
 

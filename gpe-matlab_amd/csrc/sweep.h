@@ -25,8 +25,13 @@ enum Seq : int {
   S_NSEQ = 10
 };
 
-// Reduction slots of the partial-sum buffer.
-enum Slot : int { SL_A = 0, SL_B = 1, SL_C = 2, SL_KX = 3, SL_KY = 4, SL_KZ = 5, SL_MAX = 6, SL_NSLOT = 8 };
+// Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
+// C12 for |psi1|^2|psi2|^2, C2 for |psi2|^4, X for Re conj(psi1) Omega psi2 and
+// KX2..KZ2 for the kinetic sums of the second component.
+enum Slot : int {
+  SL_A = 0, SL_B = 1, SL_C = 2, SL_KX = 3, SL_KY = 4, SL_KZ = 5, SL_MAX = 6,
+  SL_C12 = 7, SL_C2 = 8, SL_X = 9, SL_KX2 = 10, SL_KY2 = 11, SL_KZ2 = 12, SL_NSLOT = 16
+};
 
 // Scalars that evolve on the device between kernels (no host round trip).
 struct DevState {
@@ -58,10 +63,34 @@ struct SweepParams {
   int slot_ke;               // slot receiving k^2 sums
   double g;                  // interaction constant
   double dr, di;             // complex time step dt = dr + i di (ITP: dr = dt, di = 0)
+  int apply_scale;           // S_FWD: multiply by the pending scale st->scale first
+};
+
+// Two-component x-sweep parameters (GPEtaskSpinor).
+enum SpSeq : int { SP_PR = 0, SP_KEX = 1 };
+
+struct SpinorParams {
+  double2* psi1;
+  double2* psi2;
+  const double* V1;
+  const double* V2;
+  const double2* cpl;        // space-dependent coupling Omega(x), or null for a scalar
+  const double2* tw;
+  double* partials;
+  DevState* st;
+  unsigned ncols, A, sa, sb, stride;
+  int maxwg;
+  double g11, g12, g21, g22;
+  double dr, di;             // complex dt
+  double2 cpl_s;             // scalar coupling (cpl == null)
+  double2 cosom, sinomm, sinomp;  // scalar Rabi coefficients (solve_split.m:61-65), cpl == null
+  double nlincpl;            // nonlinear coupling (solve_split.m:84-89)
+  const double* keax;        // k^2/2/n along x (SP_KEX)
 };
 
 // Launch one sweep; returns hipErrorInvalidValue for an unsupported length.
 hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks);
 bool sweep_length_supported(int n);
+hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks);
 
 }  // namespace gpe

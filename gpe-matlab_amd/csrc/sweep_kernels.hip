@@ -80,6 +80,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
     }
     if constexpr (SEQ == S_MID) fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_FWD) {
+    if (p.apply_scale) {
+      const double s = p.st->scale;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cscale(v[m], s);
+    }
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_INV) {
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);

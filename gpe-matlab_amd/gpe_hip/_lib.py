@@ -25,7 +25,7 @@ class GridDesc(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("ncomp", C.c_int), ("g", C.c_double * 4), ("gamma", C.c_double), ("decay_rate", C.c_double),
-                ("Ntotal", C.c_double), ("mu_init", C.c_double), ("n_recalc", C.c_int)]
+                ("Ntotal", C.c_double), ("mu_init", C.c_double), ("n_recalc", C.c_int), ("nlincpl", C.c_double)]
 
 
 class ItpArgs(C.Structure):
@@ -148,7 +148,7 @@ class Device:
     def call(self, name, *args):
         check(getattr(self.lib, name)(self.h, *args), self.h, name)
 
-    def set_params(self, ncomp, g, gamma, decay_rate, Ntotal, mu_init, n_recalc):
+    def set_params(self, ncomp, g, gamma, decay_rate, Ntotal, mu_init, n_recalc, nlincpl=0.0):
         p = Params()
         p.ncomp = ncomp
         gm = np.asarray(g, dtype=np.float64)
@@ -160,6 +160,7 @@ class Device:
                 p.g[i] = float(flat[i])
         p.gamma, p.decay_rate, p.Ntotal, p.mu_init = float(gamma), float(decay_rate), float(Ntotal), float(mu_init)
         p.n_recalc = int(n_recalc)
+        p.nlincpl = float(nlincpl)
         self.call("gpe_set_params", C.byref(p))
 
     def set_potential(self, comp, V):

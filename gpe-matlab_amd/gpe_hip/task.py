@@ -277,3 +277,118 @@ class GPEtask:
             out.append(EE)
         self.init_state = phi                                   # 111
         return out[0] if nargout == 1 else tuple(out)
+
+
+class GPEtaskSpinor(GPEtask):
+    """Drop-in for lib/@GPEtaskSpinor (GPEtaskSpinor.m:1-146): two coherently coupled
+    components with a scalar or space-dependent complex Rabi coupling.
+
+    Deviation: the chemical potential is reduced as a real number (its imaginary
+    part is rounding noise: the coupled Hamiltonian is Hermitian)."""
+
+    _ncomp = 2
+
+    def __init__(self, grid, trappot, seed=None):
+        # GPEtaskSpinor.m:12-22
+        super().__init__(grid, trappot, seed)
+        self.coupling = 0.0
+        self.nlincpl = 0.0
+        self.ncomp = 0
+        if isinstance(trappot, (list, tuple)):
+            self.ncomp = len(trappot)
+            self.Vtrap = [t(grid.mesh.x, grid.mesh.y, grid.mesh.z) if callable(t) else t for t in trappot]
+
+    def getVtotal(self, time):
+        # GPEtaskSpinor.m:24-44
+        if self.Vtd is None:
+            return self.Vtrap
+        if time == 0 and np.size(self.V0) > 1:
+            return self.V0
+        if callable(self.Vtd):
+            return [self.Vtrap[i] + self.Vtd(self, time) for i in range(self.ncomp)]
+        if isinstance(self.Vtd, (list, tuple)):
+            return [self.Vtrap[i] + self.Vtd[i](self, time) for i in range(self.ncomp)]
+        return self.Vtrap
+
+    def _set_params(self, Ntotal_eff):
+        self._device().set_params(2, np.asarray(self.g, dtype=float).reshape(2, 2), self.gamma, self.decay_rate,
+                                  Ntotal_eff, self.mu_init, self.n_recalc, self.nlincpl)
+
+    def _fetch(self):
+        d = self._device()
+        return [d.get_state(0), d.get_state(1)]
+
+    def _upload(self, phi):
+        d = self._device()
+        d.set_state(0, phi[0])
+        d.set_state(1, phi[1])
+
+    def _upload_potential(self, V):
+        d = self._device()
+        d.set_potential(0, V[0])
+        d.set_potential(1, V[1])
+
+    def _upload_coupling(self):
+        self._device().set_coupling(self.coupling)
+
+    def get_energy(self, phi=None, time=None):
+        """GPEtaskSpinor.m:100-111 -- real <phi|H_{g/2} phi>, reduced on the device."""
+        m = self._measure_sp(phi, time)
+        return (m[1] + m[3]) + m[4] + 0.5 * m[2]
+
+    def _measure_sp(self, phi, time):
+        self._check_omega()
+        dev = self._device()
+        self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+        if time is None:
+            time = self.current_time
+        self._upload_potential(self.getVtotal(time))
+        self._upload_coupling()
+        saved = self._fetch() if (phi is not None and self._dev_state_valid) else None
+        if phi is not None:
+            self._upload(phi)
+        elif not self._dev_state_valid:
+            self._upload(self.current_state)
+            self._dev_state_valid = True
+        m = dev.measure()
+        if saved is not None:
+            self._upload(saved)
+        return m
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """@GPEtaskSpinor/solve_split.m:1-121 on the GPU; returns [psi1, psi2]."""
+        self._check_omega()
+        dev = self._device()
+        if np.asarray(self.g).shape != (2, 2):
+            raise ValueError("GPEtaskSpinor needs a 2x2 g matrix")
+        if not self.Ntotal > 0:
+            raise ValueError("GPEtaskSpinor.solve_split needs Ntotal > 0 (solve_split.m:30-38)")
+        start = self.current_iter
+        self._set_params(self.Ntotal)
+        if start > 0:
+            if not self._dev_state_valid:
+                self._upload(self.current_state)
+        else:
+            self._upload(self.init_state)
+        self._dev_state_valid = True
+        self._upload_potential(self.getVtotal(self.current_time))
+        self._upload_coupling()
+        dev.split_begin(ddt, niter_inner)
+        static = self.Vtd is None
+        n_rec = self.n_recalc
+        dt_outer = ddt * niter_inner
+        cpl = self.coupling
+        for j in range(start + 1, niter_outer + 1):
+            if self.coupling is not cpl:                       # callbacks may change it (61-65)
+                cpl = self.coupling
+                self._upload_coupling()
+            if static:
+                time2, mu, ncur = dev.split_outer(j)
+            else:
+                time = (j - 1) * dt_outer
+                for jj in range(1, niter_inner // n_rec + 1):
+                    time2 = time + (jj - 1) * ddt * n_rec
+                    self._upload_potential(self.getVtotal(time2))
+                    mu, ncur = dev.split_block(time2)
+            self.ext_callback(j, time2, mu, ncur)
+        return self.current_state

@@ -51,6 +51,7 @@ typedef struct {
   double Ntotal;      /* GPEtask.m:12 */
   double mu_init;     /* GPEtask.m:13 */
   int n_recalc;       /* GPEtask.m:9  */
+  double nlincpl;     /* GPEtaskSpinor.m:7 (density-dependent coupling), 0 otherwise */
 } gpe_params;
 
 /* grid2d/grid3d construction + GPEtask(grid, V) (GPEtask.m:41-60): allocates the

@@ -1,0 +1,77 @@
+"""GPU parity of GPEtaskSpinor.solve_split (fused 2x2 Rabi step) against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-11
+
+
+def _spinor_pair(dims, n, xmax, coupling, g, gamma=0.0, nlincpl=0.0, seed=21):
+    import gpe_hip
+    if dims == 2:
+        G, O = gpe_hip.grid2d(xmax[0], n[0], xmax[1], n[1]), oracle.grid2d(xmax[0], n[0], xmax[1], n[1])
+    else:
+        G = gpe_hip.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2])
+        O = oracle.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2])
+
+    def V(X, Y, Z):
+        return 0.5 * (X ** 2 + Y ** 2 + (Z ** 2 if dims == 3 else 0))
+
+    def V2(X, Y, Z):
+        return V(X, Y, Z) + 0.3 * X
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r2 = G.mesh.x ** 2 + G.mesh.y ** 2 + (G.mesh.z ** 2 if dims == 3 else 0)
+    a = np.exp(-r2 / 3) * np.exp(1j * rng.uniform(0, 0.2, G.size))
+    b = 0.7 * np.exp(-r2 / 2) * np.exp(1j * rng.uniform(0, 0.2, G.size))
+    tg, to = gpe_hip.GPEtaskSpinor(G, [V, V2]), oracle.GPEtaskSpinor(O, [V, V2])
+    cp = coupling(G) if callable(coupling) else coupling
+    for t in (tg, to):
+        t.g = np.array(g)
+        t.gamma = gamma
+        t.nlincpl = nlincpl
+        t.coupling = cp
+        t.Ntotal = 5.0
+        t.init_state = [a.copy(), b.copy()]
+    return tg, to
+
+
+@pytest.mark.parametrize("dims,n,xmax", [(2, (32, 24), (6, 5)), (2, (64, 64), (8, 8)), (3, (16, 24, 12), (5, 6, 4)),
+                                         (3, (48, 32, 32), (6, 5, 5))])
+@pytest.mark.parametrize("kind", ["scalar", "array", "damped", "nlin"])
+def test_spinor_parity(dims, n, xmax, kind):
+    g = [[0.5, 0.45], [0.4, 0.6]]
+    if kind == "scalar":
+        tg, to = _spinor_pair(dims, n, xmax, 0.5 * np.exp(0.3j), g)
+    elif kind == "array":
+        tg, to = _spinor_pair(dims, n, xmax, lambda G: 0.5 * np.exp(2j * G.mesh.x), g)
+    elif kind == "damped":
+        tg, to = _spinor_pair(dims, n, xmax, lambda G: 0.5 * np.exp(2j * G.mesh.x), g, gamma=0.02)
+    else:
+        tg, to = _spinor_pair(dims, n, xmax, 0.4, g, nlincpl=0.05)
+    pg = tg.solve_split(0.002, 20, 2)
+    po = to.solve_split(0.002, 20, 2)
+    err = max(rel_l2(pg[0], po[0]), rel_l2(pg[1], po[1]))
+    print(f"spinor {dims}D {n} {kind}: rel-L2 = {err:.3e}")
+    assert err < TOL
+    assert np.allclose(tg.history["mu"], np.real(to.history["mu"]), rtol=1e-11, atol=1e-13)
+    assert np.all(np.abs(np.imag(to.history["mu"])) < 1e-12 * np.abs(to.history["mu"]))
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
+
+
+def test_spinor_rabi_closed_form_gpu():
+    """V=0, g=0, uniform real Omega: imbalance (n1-n2)/N = cos(2 Omega t) exactly."""
+    import gpe_hip
+    grid = gpe_hip.grid2d(5, 16, 5, 16)
+    t = gpe_hip.GPEtaskSpinor(grid, [0 * grid.mesh.x, 0 * grid.mesh.x])
+    t.g = np.zeros((2, 2))
+    t.coupling = 0.5
+    t.Ntotal = grid.integrate(np.ones(grid.size))
+    t.init_state = [np.ones(grid.size, complex), np.zeros(grid.size, complex)]
+    out = t.solve_split(0.01, 30, 4)
+    n1 = grid.integrate(np.abs(out[0]) ** 2)
+    n2 = grid.integrate(np.abs(out[1]) ** 2)
+    assert abs((n1 - n2) / t.Ntotal - np.cos(2 * 0.5 * 1.2)) < 1e-12
