@@ -117,31 +117,40 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
 
 int gpe_destroy(gpe_handle* h) {
   if (!h) return GPE_OK;
-  hipSetDevice(h->device);
+  (void)hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
   for (int c = 0; c < 2; ++c) {
-    hipFree(h->psi[c]);
-    hipFree(h->V[c]);
+    (void)hipFree(h->psi[c]);
+    (void)hipFree(h->V[c]);
   }
   for (int a = 0; a < 3; ++a) {
-    hipFree(h->ax[a].tw);
-    hipFree(h->ax[a].keax);
+    (void)hipFree(h->ax[a].tw);
+    (void)hipFree(h->ax[a].keax);
   }
-  hipFree(h->cpl);
-  hipFree(h->kcol);
-  hipFree(h->kax);
+  (void)hipFree(h->cpl);
+  (void)hipFree(h->kcol);
+  (void)hipFree(h->kax);
   for (int i = 0; i < 2; ++i) {
-    hipFree(h->kcol_sp[i]);
-    hipFree(h->kax_sp[i]);
+    (void)hipFree(h->kcol_sp[i]);
+    (void)hipFree(h->kax_sp[i]);
   }
-  hipFree(h->nop);
-  hipFree(h->psi_old);
-  hipFree(h->partials);
-  hipFree(h->st);
-  hipFree(h->MUdev);
+  (void)hipFree(h->nop);
+  (void)hipFree(h->psi_old);
+  (void)hipFree(h->adj_V0);
+  (void)hipFree(h->adj_G);
+  (void)hipFree(h->adj_lam);
+  (void)hipFree(h->adj_gV);
+  (void)hipFree(h->adj_ckpt);
+  (void)hipFree(h->adj_target);
+  (void)hipFree(h->adj_grad);
+  (void)hipFree(h->kcol_c);
+  (void)hipFree(h->kax_c);
+  (void)hipFree(h->partials);
+  (void)hipFree(h->st);
+  (void)hipFree(h->MUdev);
   for (auto& r : h->prof_recs) {
-    hipEventDestroy(r.second.first);
-    hipEventDestroy(r.second.second);
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
   }
   for (hipEvent_t e : h->ev_pool) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -317,7 +326,7 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
   if (!h->nop) HIPCHK(h, hipMalloc(&h->nop, h->npts * sizeof(double)));
   if (!h->psi_old) HIPCHK(h, hipMalloc(&h->psi_old, h->npts * sizeof(double2)));
   if (h->MUcap < a->max_iter) {
-    hipFree(h->MUdev);
+    (void)hipFree(h->MUdev);
     h->MUdev = nullptr;
     HIPCHK(h, hipMalloc(&h->MUdev, a->max_iter * sizeof(double)));
     h->MUcap = a->max_iter;
@@ -450,9 +459,9 @@ int gpe_selftest_phase(int n, const double* er, const double* ei, double* out) {
       gpe::launch_phase_selftest(der, dei, dout, n, nullptr) == hipSuccess &&
       hipMemcpy(out, dout, n * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess)
     rc = GPE_OK;
-  hipFree(der);
-  hipFree(dei);
-  hipFree(dout);
+  (void)hipFree(der);
+  (void)hipFree(dei);
+  (void)hipFree(dout);
   return rc;
 }
 

@@ -66,6 +66,19 @@ struct gpe_handle {
   int niter_inner = 0, n_rec = 1;
   double NN0 = 0;
   std::string err;
+  // adjoint / control (gpe_control_set, gpe_adjoint_grad)
+  int adj_nbasis = 0;
+  double* adj_V0 = nullptr;
+  double* adj_G = nullptr;        // nbasis x npts basis fields
+  double2* adj_lam = nullptr;
+  double* adj_gV = nullptr;
+  double2* adj_ckpt = nullptr;
+  int adj_nckpt = 0;
+  double2* adj_target = nullptr;
+  double* adj_grad = nullptr;
+  int adj_gradcap = 0;
+  double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
+  double2* kax_c = nullptr;
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;

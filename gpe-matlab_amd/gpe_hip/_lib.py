@@ -28,6 +28,11 @@ class Params(C.Structure):
                 ("Ntotal", C.c_double), ("mu_init", C.c_double), ("n_recalc", C.c_int), ("nlincpl", C.c_double)]
 
 
+class ControlArgs(C.Structure):
+    _fields_ = [("ddt", C.c_double), ("nsteps", C.c_int), ("steps_per_interval", C.c_int), ("nintervals", C.c_int),
+                ("checkpoint_blocks", C.c_int)]
+
+
 class ItpArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("min_iter", C.c_int),
                 ("adjust_stepsize", C.c_int), ("nargout", C.c_int)]
@@ -55,6 +60,9 @@ SIGNATURES = {
     "gpe_profile_enable": [_P, C.c_int],
     "gpe_profile_read": [_P, _DP, _IP, C.c_int],
     "gpe_selftest_phase": [C.c_int, _DP, _DP, _DP],
+    "gpe_control_set": [_P, C.c_int, _DP, _DP],
+    "gpe_control_forward": [_P, C.POINTER(ControlArgs), _DP, _DP],
+    "gpe_adjoint_grad": [_P, C.POINTER(ControlArgs), _DP, _DP, _DP, _DP, _DP],
 }
 OPTIONAL = set()
 
@@ -213,6 +221,34 @@ class Device:
         self.call("gpe_itp", C.byref(a), dptr(MU), dptr(MU2), dptr(EE), C.byref(nit), C.byref(dtf), C.byref(conv))
         i = nit.value
         return MU[:i], MU2[:i], EE[:i], i, dtf.value, bool(conv.value)
+
+    def control_set(self, V0, basis):
+        v0 = to_colmajor_f64(np.broadcast_to(V0, self.shape))
+        G = np.concatenate([to_colmajor_f64(np.broadcast_to(b, self.shape)) for b in basis])
+        self.call("gpe_control_set", len(basis), dptr(v0), dptr(G))
+
+    @staticmethod
+    def _cargs(ddt, nsteps, spi, nint, ckpt):
+        return ControlArgs(float(ddt), int(nsteps), int(spi), int(nint), int(ckpt))
+
+    def control_forward(self, ddt, nsteps, spi, coeffs, ckpt=0):
+        c = np.asfortranarray(coeffs, dtype=np.float64)
+        a = self._cargs(ddt, nsteps, spi, c.shape[1], ckpt)
+        out = np.empty(self.npts, dtype=np.complex128)
+        self.call("gpe_control_forward", C.byref(a), c.ravel(order="F").ctypes.data_as(_DP),
+                  out.view(np.float64).ctypes.data_as(_DP))
+        return from_colmajor_c128(out, self.shape)
+
+    def adjoint_grad(self, ddt, nsteps, spi, coeffs, psi_target, ckpt=0):
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64).ravel(order="F"))
+        nint = np.asarray(coeffs).shape[1]
+        a = self._cargs(ddt, nsteps, spi, nint, ckpt)
+        t = to_colmajor_c128(np.broadcast_to(psi_target, self.shape))
+        loss, rerr = C.c_double(), C.c_double()
+        grad = np.zeros(c.size)
+        self.call("gpe_adjoint_grad", C.byref(a), dptr(c), t.view(np.float64).ctypes.data_as(_DP), C.byref(loss),
+                  dptr(grad), C.byref(rerr))
+        return loss.value, grad.reshape(np.asarray(coeffs).shape, order="F"), rerr.value
 
     def profile_enable(self, on=True):
         self.call("gpe_profile_enable", int(bool(on)))

@@ -113,6 +113,30 @@ typedef struct {
 int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, double* EE, int* n_iter,
             double* dt_final, int* converged);
 
+/* ------------------------------------------------------------------ adjoint
+ * (no reference counterpart: north-star form 3, the gradient entry point that
+ * plays the torch.autograd.Function.backward role).
+ * Control potential V(x, t) = V0(x) + sum_k c[k + nbasis*m] G_k(x), where m is the
+ * interval of the block holding t (V is evaluated at block starts,
+ * solve_split.m:54; intervals are steps_per_interval steps long).  The forward
+ * model is GPEtask.solve_split with gamma = 0; the loss is
+ * 1 - |<psi_target|psi(T)>|^2.  Both calls start from the uploaded state
+ * (gpe_set_state) and leave it in place. */
+typedef struct {
+  double ddt;
+  int nsteps;              /* T, a multiple of n_recalc                        */
+  int steps_per_interval;  /* a multiple of n_recalc                           */
+  int nintervals;          /* number of coefficient columns m                  */
+  int checkpoint_blocks;   /* blocks between checkpoints, 0: about sqrt(T) steps */
+} gpe_control_args;
+int gpe_control_set(gpe_handle* h, int nbasis, const double* V0, const double* G);
+int gpe_control_forward(gpe_handle* h, const gpe_control_args* a, const double* coeffs, double* psi_out);
+/* loss, grad[nbasis*nintervals] (dL/dc, same layout as coeffs), and the relative
+ * L2 difference between the backward-reconstructed initial state and the stored
+ * one (recon_err, may be NULL). */
+int gpe_adjoint_grad(gpe_handle* h, const gpe_control_args* a, const double* coeffs, const double* psi_target,
+                     double* loss, double* grad, double* recon_err);
+
 /* Measurement hooks (no reference counterpart): time every sweep launch with
  * HIP events recorded on the handle's own stream.  gpe_profile_read fills
  * ms[seq*3 + axis] (summed milliseconds) and count[seq*3 + axis] for
