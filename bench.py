@@ -60,7 +60,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
+
+    if args.copy_calib:
+        import torch
+        x = torch.empty(16 * 1024 * 1024, dtype=torch.complex128, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": x.numel() * 16, "copy_calib_bytes_written": x.numel() * 16}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
