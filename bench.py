@@ -38,6 +38,17 @@ SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 
 
+def committed_traffic(n):
+    """HBM bytes per launch per sweep from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", f"sweep{n}_c2_hbm_traffic.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].strip().split(", "))}
+    return {}
+
+
 def sync():
     try:
         import torch
@@ -161,9 +172,12 @@ def main():
     for k in kernels:
         log(f"  {k['kernel']:8s} x{k['launches']:3d} avg {k['avg_ms'] * 1e3:8.1f} us  {k['alg_GBps']:7.1f} GB/s alg")
     dom = kernels[0]
+    traffic = committed_traffic(args.n).get(dom["kernel"])
     roofline = {"bound": "hbm", "achieved": round(dom["alg_GBps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                "frac": round(dom["alg_GBps"] / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": dom["kernel"],
-                "alg_bytes_per_launch": dom["alg_bytes"]}
+                "frac": round(dom["alg_GBps"] / PEAK_HBM_GBPS, 4), "traffic": traffic, "kernel": dom["kernel"],
+                "alg_bytes_per_launch": dom["alg_bytes"],
+                "traffic_source": "profiles/sweep256_c2_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
+                if traffic else None}
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
 
     out = {
