@@ -52,6 +52,7 @@ def build(jobs=None, verbose=True):
         tasks.append(("sweep_kernels.hip", os.path.join(BUILD, f"sweep_{n}.o"), (f"-DGPE_N={n}",)))
         tasks.append(("spinor_kernels.hip", os.path.join(BUILD, f"spinor_{n}.o"), (f"-DGPE_N={n}",)))
         tasks.append(("adjoint_kernels.hip", os.path.join(BUILD, f"adjoint_{n}.o"), (f"-DGPE_N={n}",)))
+        tasks.append(("batch_kernels.hip", os.path.join(BUILD, f"batch_{n}.o"), (f"-DGPE_N={n}",)))
     for s in SOURCES:
         if os.path.exists(os.path.join(CSRC, s)):
             tasks.append((s, os.path.join(BUILD, s.replace(".hip", ".o")), ()))

@@ -1,0 +1,466 @@
+// Batched ragged ensemble driver: many independent 2D GPEtask.solve_split runs
+// (different grid sizes, potentials, g, gamma) advanced together.
+//
+// Every sweep of a block is one launch per distinct pencil length across all
+// trajectories (work items = (trajectory, pencil tile)), so a block of n_rec
+// steps costs 2 n_rec + 5 launches per length group instead of per trajectory.
+// With decay_rate = 0 everywhere, one whole outer iteration (niter_inner steps)
+// is captured once into a hipGraph and replayed.  Per-trajectory scalars (mu,
+// renormalisation, ncur) live in a DevState array and are reduced on the device.
+#include <algorithm>
+#include <map>
+
+#include "batch.h"
+#include "finalize_dev.h"
+#include "handle.h"
+
+using namespace gpe_detail;
+
+namespace gpe {
+
+__global__ void __launch_bounds__(256) batch_finalize_kernel(const TrajDev* traj, int mode, double time2) {
+  __shared__ double red[4];
+  const TrajDev& tj = traj[blockIdx.x];
+  FinParams f{};
+  f.partials = tj.partials;
+  f.maxwg = tj.maxwg;
+  f.cnt[SL_A] = f.cnt[SL_B] = f.cnt[SL_C] = f.cnt[SL_KX] = tj.tiles_x;
+  f.cnt[SL_KY] = tj.tiles_y;
+  f.st = tj.st;
+  f.mode = mode;
+  f.renorm = tj.renorm;
+  f.w = tj.w;
+  f.g = tj.g;
+  f.NN0 = tj.NN0;
+  f.tau = tj.tau;
+  f.time2 = time2;
+  finalize_core(f, red);
+}
+
+hipError_t launch_batch_finalize(const TrajDev* traj, int ntraj, int mode, double time2, hipStream_t s) {
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)ntraj), dim3(256), 0, s, traj, mode, time2);
+  return hipGetLastError();
+}
+
+#define GPE_DECL(N) hipError_t launch_batch_sweep_##N(bool, int, const BatchLaunch&, hipStream_t);
+GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
+GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
+#undef GPE_DECL
+
+hipError_t launch_batch_sweep(int n, bool xaxis, int seq, const BatchLaunch& b, hipStream_t s) {
+  switch (n) {
+#define GPE_C(N) case N: return launch_batch_sweep_##N(xaxis, seq, b, s);
+    GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
+    GPE_C(256) GPE_C(384) GPE_C(512)
+#undef GPE_C
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gpe
+
+struct gpe_batch {
+  struct Traj {
+    int nx = 0, ny = 0;
+    double dx = 0, dy = 0, w = 0;
+    std::vector<double> kx, ky;
+    gpe_params prm{};
+    bool set = false;
+    size_t off = 0, koff_x = 0, koff_y = 0;
+    int tiles_x = 0, tiles_y = 0;
+  };
+  struct Group {
+    int n = 0;
+    size_t first = 0;  // offset into the device item array
+    int count = 0;
+  };
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Traj> tr;
+  size_t total = 0, ktotal_x = 0, ktotal_y = 0;
+  double2* psi = nullptr;
+  double* V = nullptr;
+  double2* kcol = nullptr;
+  double2* kax = nullptr;
+  double* keax_x = nullptr;
+  double* keax_y = nullptr;
+  std::map<int, double2*> tw;
+  DevState* st = nullptr;
+  double* partials = nullptr;
+  int maxwg = 0;
+  gpe::TrajDev* d_traj = nullptr;
+  std::vector<gpe::TrajDev> h_traj;
+  gpe::WorkItem* d_items = nullptr;
+  std::vector<Group> gx, gy;
+  double ddt = 0;
+  int niter_inner = 0, n_rec = 1;
+  bool begun = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int bfail(gpe_batch* b, int code, const char* fmt, ...) {
+  if (b) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    b->err = buf;
+  }
+  return code;
+}
+
+#define BCHK(b, x)                                                                            \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess) return bfail((b), GPE_EHIP, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+int bsweep(gpe_batch* b, bool xaxis, int seq) {
+  for (const auto& g : (xaxis ? b->gx : b->gy)) {
+    gpe::BatchLaunch L{b->d_traj, b->d_items + g.first, g.count};
+    hipError_t e = gpe::launch_batch_sweep(g.n, xaxis, seq, L, b->stream);
+    if (e != hipSuccess) return bfail(b, GPE_EHIP, "batch sweep (n %d, seq %d): %s", g.n, seq, hipGetErrorString(e));
+  }
+  return GPE_OK;
+}
+
+int bmeasure(gpe_batch* b, int mode, double time2) {
+  int rc;
+  if ((rc = bsweep(b, true, gpe::S_KEX))) return rc;
+  if ((rc = bsweep(b, false, gpe::S_KE))) return rc;
+  BCHK(b, gpe::launch_batch_finalize(b->d_traj, (int)b->tr.size(), mode, time2, b->stream));
+  return GPE_OK;
+}
+
+// solve_split.m:53-78 for every trajectory at once
+int bblock(gpe_batch* b, double time2) {
+  int rc;
+  for (int i = 0; i < b->n_rec; ++i) {
+    if ((rc = bsweep(b, true, i == 0 ? gpe::S_OPEN : gpe::S_MID))) return rc;
+    if ((rc = bsweep(b, false, gpe::S_KIN))) return rc;
+  }
+  if ((rc = bsweep(b, true, gpe::S_CLOSE))) return rc;
+  return bmeasure(b, gpe::FIN_BLOCK, time2);
+}
+
+int bouter(gpe_batch* b, int j) {
+  int rc;
+  const double dt_outer = b->ddt * b->niter_inner;
+  const double time = (double)(j - 1) * dt_outer;
+  for (int jj = 1; jj <= b->niter_inner / b->n_rec; ++jj) {
+    const double time2 = time + (double)(jj - 1) * b->ddt * (double)b->n_rec;
+    if ((rc = bblock(b, time2))) return rc;
+  }
+  return GPE_OK;
+}
+
+void drop_graph(gpe_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gpe_batch_last_error(const gpe_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int gpe_batch_destroy(gpe_batch* b) {
+  if (!b) return GPE_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  (void)hipFree(b->psi);
+  (void)hipFree(b->V);
+  (void)hipFree(b->kcol);
+  (void)hipFree(b->kax);
+  (void)hipFree(b->keax_x);
+  (void)hipFree(b->keax_y);
+  for (auto& kv : b->tw) (void)hipFree(kv.second);
+  (void)hipFree(b->st);
+  (void)hipFree(b->partials);
+  (void)hipFree(b->d_traj);
+  (void)hipFree(b->d_items);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return GPE_OK;
+}
+
+int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
+  if (!out || !grids || ntraj < 1) return GPE_EINVAL;
+  *out = nullptr;
+  for (int i = 0; i < ntraj; ++i) {
+    const gpe_grid_desc& d = grids[i];
+    if (d.ndims != 2 || !gpe::sweep_length_supported(d.n[0]) || !gpe::sweep_length_supported(d.n[1]) ||
+        !(d.xmax[0] > 0) || !(d.xmax[1] > 0))
+      return GPE_EINVAL;
+  }
+  gpe_batch* b = new gpe_batch();
+  const char* dev = getenv("GPE_DEVICE");
+  b->device = dev ? atoi(dev) : 0;
+  if (hipSetDevice(b->device) != hipSuccess) {
+    delete b;
+    return GPE_EHIP;
+  }
+  b->tr.resize(ntraj);
+  int maxwg = 1;
+  for (int i = 0; i < ntraj; ++i) {
+    auto& t = b->tr[i];
+    t.nx = grids[i].n[0];
+    t.ny = grids[i].n[1];
+    std::vector<double> x = linspace(-grids[i].xmax[0], grids[i].xmax[0], t.nx);
+    std::vector<double> y = linspace(-grids[i].xmax[1], grids[i].xmax[1], t.ny);
+    t.dx = x[1] - x[0];
+    t.dy = y[1] - y[0];
+    t.w = t.dx * t.dy;
+    t.kx = kvec(x, t.dx);
+    t.ky = kvec(y, t.dy);
+    t.off = b->total;
+    b->total += (size_t)t.nx * t.ny;
+    t.koff_x = b->ktotal_x;
+    b->ktotal_x += t.nx;
+    t.koff_y = b->ktotal_y;
+    b->ktotal_y += t.ny;
+    t.tiles_x = (t.ny + gpe::sweep_cols_per_block(t.nx) - 1) / gpe::sweep_cols_per_block(t.nx);
+    t.tiles_y = (t.nx + gpe::sweep_cols_per_block(t.ny) - 1) / gpe::sweep_cols_per_block(t.ny);
+    maxwg = std::max(maxwg, std::max(t.tiles_x, t.tiles_y));
+  }
+  b->maxwg = maxwg;
+  // work items grouped by swept length
+  std::vector<gpe::WorkItem> items;
+  for (int axis = 0; axis < 2; ++axis) {
+    std::map<int, std::vector<gpe::WorkItem>> by_n;
+    for (int i = 0; i < ntraj; ++i) {
+      const auto& t = b->tr[i];
+      const int n = axis == 0 ? t.nx : t.ny;
+      const int tiles = axis == 0 ? t.tiles_x : t.tiles_y;
+      for (int k = 0; k < tiles; ++k) by_n[n].push_back({i, k});
+    }
+    for (auto& kv : by_n) {
+      gpe_batch::Group g;
+      g.n = kv.first;
+      g.first = items.size();
+      g.count = (int)kv.second.size();
+      items.insert(items.end(), kv.second.begin(), kv.second.end());
+      (axis == 0 ? b->gx : b->gy).push_back(g);
+    }
+  }
+  auto bail = [&](int code) {
+    gpe_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(GPE_EHIP);
+  if (hipMalloc(&b->psi, b->total * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->V, b->total * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->kcol, b->ktotal_x * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->kax, b->ktotal_y * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->keax_x, b->ktotal_x * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->keax_y, b->ktotal_y * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->st, ntraj * sizeof(DevState)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->partials, (size_t)ntraj * gpe::SL_NSLOT * maxwg * sizeof(double)) != hipSuccess)
+    return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->d_traj, ntraj * sizeof(gpe::TrajDev)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMalloc(&b->d_items, items.size() * sizeof(gpe::WorkItem)) != hipSuccess) return bail(GPE_ENOMEM);
+  if (hipMemcpy(b->d_items, items.data(), items.size() * sizeof(gpe::WorkItem), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(GPE_EHIP);
+  std::vector<double> kex(b->ktotal_x), key(b->ktotal_y);
+  for (auto& t : b->tr) {
+    for (int n : {t.nx, t.ny})
+      if (!b->tw.count(n)) {
+        double2* p = nullptr;
+        std::vector<double2> tw = twiddles(n);
+        if (hipMalloc(&p, n * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
+        b->tw[n] = p;
+        if (hipMemcpy(p, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP);
+      }
+    for (int i = 0; i < t.nx; ++i) kex[t.koff_x + i] = t.kx[i] * t.kx[i] / 2.0 / (double)t.nx;
+    for (int i = 0; i < t.ny; ++i) key[t.koff_y + i] = t.ky[i] * t.ky[i] / 2.0 / (double)t.ny;
+  }
+  if (hipMemcpy(b->keax_x, kex.data(), kex.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->keax_y, key.data(), key.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(GPE_EHIP);
+  std::vector<DevState> s0(ntraj);
+  for (auto& s : s0) {
+    s = DevState{};
+    s.scale = 1.0;
+    s.mu_itp = 1.0;
+  }
+  if (hipMemcpy(b->st, s0.data(), ntraj * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP);
+  b->h_traj.resize(ntraj);
+  *out = b;
+  return GPE_OK;
+}
+
+int gpe_batch_set(gpe_batch* b, int i, const gpe_params* p, const double* V, const double* psi) {
+  if (!b || !p || i < 0 || i >= (int)b->tr.size()) return b ? bfail(b, GPE_EINVAL, "bad trajectory") : GPE_EINVAL;
+  if (p->ncomp != 1 || p->n_recalc < 1) return bfail(b, GPE_EINVAL, "one component, n_recalc >= 1");
+  BCHK(b, hipSetDevice(b->device));
+  auto& t = b->tr[i];
+  const size_t n = (size_t)t.nx * t.ny;
+  if (V) BCHK(b, hipMemcpyAsync(b->V + t.off, V, n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (psi) BCHK(b, hipMemcpyAsync(b->psi + t.off, psi, n * sizeof(double2), hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  t.prm = *p;
+  t.set = true;
+  b->begun = false;
+  drop_graph(b);
+  if (psi) {
+    DevState s;
+    BCHK(b, hipMemcpy(&s, b->st + i, sizeof s, hipMemcpyDeviceToHost));
+    s.scale = 1.0;
+    BCHK(b, hipMemcpy(b->st + i, &s, sizeof s, hipMemcpyHostToDevice));
+  }
+  return GPE_OK;
+}
+
+int gpe_batch_get_state(gpe_batch* b, int i, double* psi) {
+  if (!b || !psi || i < 0 || i >= (int)b->tr.size()) return b ? bfail(b, GPE_EINVAL, "bad trajectory") : GPE_EINVAL;
+  BCHK(b, hipSetDevice(b->device));
+  auto& t = b->tr[i];
+  const size_t n = (size_t)t.nx * t.ny;
+  DevState s;
+  BCHK(b, hipMemcpyAsync(&s, b->st + i, sizeof s, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipMemcpyAsync(psi, b->psi + t.off, n * sizeof(double2), hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  if (s.scale != 1.0)
+    for (size_t k = 0; k < 2 * n; ++k) psi[k] = psi[k] * s.scale;
+  return GPE_OK;
+}
+
+int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_graph, double* muc_out) {
+  if (!b || niter_inner < 1) return b ? bfail(b, GPE_EINVAL, "niter_inner must be >= 1") : GPE_EINVAL;
+  BCHK(b, hipSetDevice(b->device));
+  const int ntraj = (int)b->tr.size();
+  const int nrec0 = b->tr[0].prm.n_recalc;
+  bool decay = false;
+  for (auto& t : b->tr) {
+    if (!t.set) return bfail(b, GPE_ESTATE, "every trajectory needs gpe_batch_set");
+    if (t.prm.n_recalc != nrec0) return bfail(b, GPE_EINVAL, "all trajectories need the same n_recalc");
+    if (!(t.prm.Ntotal > 0)) return bfail(b, GPE_EINVAL, "batched trajectories need Ntotal > 0");
+    decay = decay || t.prm.decay_rate > 0;
+  }
+  drop_graph(b);
+  // materialise pending renormalisations of a previous run
+  std::vector<DevState> s(ntraj);
+  BCHK(b, hipMemcpy(s.data(), b->st, ntraj * sizeof(DevState), hipMemcpyDeviceToHost));
+  for (int i = 0; i < ntraj; ++i)
+    if (s[i].scale != 1.0) {
+      auto& t = b->tr[i];
+      const size_t n = (size_t)t.nx * t.ny;
+      std::vector<double2> h(n);
+      BCHK(b, hipMemcpy(h.data(), b->psi + t.off, n * sizeof(double2), hipMemcpyDeviceToHost));
+      for (auto& v : h) v = make_double2(v.x * s[i].scale, v.y * s[i].scale);
+      BCHK(b, hipMemcpy(b->psi + t.off, h.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+      s[i].scale = 1.0;
+    }
+  b->n_rec = niter_inner % nrec0 ? 1 : nrec0;                   // solve_split.m:24-27
+  b->niter_inner = niter_inner;
+  b->ddt = ddt;
+  std::vector<double2> kc(b->ktotal_x), ka(b->ktotal_y);
+  for (int i = 0; i < ntraj; ++i) {
+    auto& t = b->tr[i];
+    const std::complex<double> dt = cdiv({0.0, ddt}, {1.0, t.prm.gamma});   // solve_split.m:31
+    for (int k = 0; k < t.nx; ++k) {
+      const std::complex<double> e = std::exp(-(t.kx[k] * t.kx[k] / 2) * dt);
+      kc[t.koff_x + k] = make_double2(e.real(), e.imag());
+    }
+    const double inv = 1.0 / ((double)t.nx * t.ny);
+    for (int k = 0; k < t.ny; ++k) {
+      const std::complex<double> e = std::exp(-(t.ky[k] * t.ky[k] / 2) * dt) * inv;
+      ka[t.koff_y + k] = make_double2(e.real(), e.imag());
+    }
+    gpe::TrajDev& d = b->h_traj[i];
+    d = gpe::TrajDev{};
+    d.psi = b->psi + t.off;
+    d.V = b->V + t.off;
+    d.kcol = b->kcol + t.koff_x;
+    d.kax = b->kax + t.koff_y;
+    d.keax_x = b->keax_x + t.koff_x;
+    d.keax_y = b->keax_y + t.koff_y;
+    d.tw_x = b->tw[t.nx];
+    d.tw_y = b->tw[t.ny];
+    d.st = b->st + i;
+    d.partials = b->partials + (size_t)i * gpe::SL_NSLOT * b->maxwg;
+    d.nx = (unsigned)t.nx;
+    d.ny = (unsigned)t.ny;
+    d.maxwg = b->maxwg;
+    d.tiles_x = t.tiles_x;
+    d.tiles_y = t.tiles_y;
+    d.renorm = (t.prm.gamma > 0 && t.prm.mu_init == 0) ? 1 : 0;
+    d.g = t.prm.g[0];
+    d.dr = dt.real();
+    d.di = dt.imag();
+    d.w = t.w;
+    d.NN0 = t.prm.Ntotal;
+    d.tau = t.prm.decay_rate;
+  }
+  BCHK(b, hipMemcpy(b->kcol, kc.data(), kc.size() * sizeof(double2), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->kax, ka.data(), ka.size() * sizeof(double2), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->d_traj, b->h_traj.data(), ntraj * sizeof(gpe::TrajDev), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->st, s.data(), ntraj * sizeof(DevState), hipMemcpyHostToDevice));
+  // solve_split.m:40-46: initial muc of every trajectory
+  int rc;
+  if ((rc = bmeasure(b, gpe::FIN_SUMS, 0.0))) return rc;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  BCHK(b, hipMemcpy(s.data(), b->st, ntraj * sizeof(DevState), hipMemcpyDeviceToHost));
+  for (int i = 0; i < ntraj; ++i) {
+    const auto& t = b->tr[i];
+    const double muc = ((s[i].sB + s[i].sK) + t.prm.g[0] * s[i].sC) * t.w / t.prm.Ntotal;
+    s[i].mu = t.prm.mu_init > 0 ? t.prm.mu_init : muc;
+    s[i].scale = 1.0;
+    s[i].NNN = t.prm.Ntotal;
+    s[i].muc = muc;
+    s[i].ncur = t.prm.Ntotal;
+    if (muc_out) muc_out[i] = muc;
+  }
+  BCHK(b, hipMemcpy(b->st, s.data(), ntraj * sizeof(DevState), hipMemcpyHostToDevice));
+  if (use_graph && !decay) {
+    // capture one outer iteration; block times only matter when decay_rate > 0
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = bouter(b, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return bfail(b, GPE_EHIP, "graph capture: %s", hipGetErrorString(ec));
+    b->graph = g;
+    BCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  }
+  b->begun = true;
+  return GPE_OK;
+}
+
+int gpe_batch_split_outer(gpe_batch* b, int j, double* time2_out, double* muc, double* ncur) {
+  if (!b) return GPE_EINVAL;
+  if (!b->begun) return bfail(b, GPE_ESTATE, "gpe_batch_split_begin has not been called");
+  BCHK(b, hipSetDevice(b->device));
+  int rc;
+  if (b->gexec) {
+    BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+  } else if ((rc = bouter(b, j))) {
+    return rc;
+  }
+  const int ntraj = (int)b->tr.size();
+  std::vector<DevState> s(ntraj);
+  BCHK(b, hipMemcpyAsync(s.data(), b->st, ntraj * sizeof(DevState), hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  const double time = (double)(j - 1) * (b->ddt * b->niter_inner);
+  if (time2_out) *time2_out = time + (double)(b->niter_inner / b->n_rec - 1) * b->ddt * (double)b->n_rec;
+  for (int i = 0; i < ntraj; ++i) {
+    if (muc) muc[i] = s[i].muc;
+    if (ncur) ncur[i] = s[i].ncur;
+  }
+  return GPE_OK;
+}
+
+}  // extern "C"

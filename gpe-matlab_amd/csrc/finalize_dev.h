@@ -1,0 +1,98 @@
+// Second-stage reductions and block-boundary scalar updates, shared by the
+// single-grid and the batched ensemble drivers.
+#pragma once
+#include "reduce_dev.h"
+#include "state.h"
+
+namespace gpe {
+
+// Sum `count` partials of one slot in a fixed order (valid on thread 0).
+__device__ __forceinline__ double slot_sum(const double* part, int maxwg, int slot, int count, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) acc += part[slot * maxwg + i];
+  return block_sum(acc, red);
+}
+
+__device__ __forceinline__ double slot_max(const double* part, int maxwg, int slot, int count, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) acc = fmax(acc, part[slot * maxwg + i]);
+  return block_max(acc, red);
+}
+
+// Second-stage reductions and the block-boundary scalar update of one grid
+// (one 256-thread work-group; `red` is a 4-double __shared__ scratch).
+__device__ __forceinline__ void finalize_core(const FinParams& f, double* red) {
+  DevState* st = f.st;
+  if (f.mode == FIN_MAX) {
+    const double m = slot_max(f.partials, f.maxwg, SL_MAX, f.cnt[SL_MAX], red);
+    if (threadIdx.x == 0) st->delta = m / (f.dt * 10.0);   // groundstate_itp.m:61
+    return;
+  }
+  const double sA = f.cnt[SL_A] ? slot_sum(f.partials, f.maxwg, SL_A, f.cnt[SL_A], red) : 0.0;
+  const double sB = f.cnt[SL_B] ? slot_sum(f.partials, f.maxwg, SL_B, f.cnt[SL_B], red) : 0.0;
+  const double sC = f.cnt[SL_C] ? slot_sum(f.partials, f.maxwg, SL_C, f.cnt[SL_C], red) : 0.0;
+  const double kx = f.cnt[SL_KX] ? slot_sum(f.partials, f.maxwg, SL_KX, f.cnt[SL_KX], red) : 0.0;
+  const double ky = f.cnt[SL_KY] ? slot_sum(f.partials, f.maxwg, SL_KY, f.cnt[SL_KY], red) : 0.0;
+  const double kz = f.cnt[SL_KZ] ? slot_sum(f.partials, f.maxwg, SL_KZ, f.cnt[SL_KZ], red) : 0.0;
+  const double c12 = f.cnt[SL_C12] ? slot_sum(f.partials, f.maxwg, SL_C12, f.cnt[SL_C12], red) : 0.0;
+  const double c2 = f.cnt[SL_C2] ? slot_sum(f.partials, f.maxwg, SL_C2, f.cnt[SL_C2], red) : 0.0;
+  const double sx = f.cnt[SL_X] ? slot_sum(f.partials, f.maxwg, SL_X, f.cnt[SL_X], red) : 0.0;
+  const double kx2 = f.cnt[SL_KX2] ? slot_sum(f.partials, f.maxwg, SL_KX2, f.cnt[SL_KX2], red) : 0.0;
+  const double ky2 = f.cnt[SL_KY2] ? slot_sum(f.partials, f.maxwg, SL_KY2, f.cnt[SL_KY2], red) : 0.0;
+  const double kz2 = f.cnt[SL_KZ2] ? slot_sum(f.partials, f.maxwg, SL_KZ2, f.cnt[SL_KZ2], red) : 0.0;
+  if (threadIdx.x != 0) return;
+  const double sK = ((kx + ky) + kz) + ((kx2 + ky2) + kz2);
+  st->sA = sA; st->sB = sB; st->sK = sK;
+  // scalar runs: sC = sum |psi|^4; spinor runs: sC = sum_jk g_jk n_j n_k, sX = sum Re conj(psi1) Omega psi2
+  st->sC = f.mode == FIN_SPINOR || f.cnt[SL_C12] ? (f.g11 * sC + (f.g12 + f.g21) * c12) + f.g22 * c2 : sC;
+  st->pad[0] = sx;
+  const double w = f.w;
+  if (f.mode == FIN_BLOCK) {
+    // solve_split.m:65-78
+    const double ncur = sA * w;
+    double muc;
+    if (f.renorm) {
+      double NNN = st->NNN;
+      if (f.tau > 0) NNN = f.NN0 * exp(-f.time2 / f.tau);
+      const double q = NNN / ncur;
+      st->scale = sqrt(q);
+      // real(inner(phi_s, applyham(phi_s)))/NNN with phi_s = sqrt(q) phi
+      muc = ((q * sB + q * sK) + f.g * (q * q) * sC) * w / NNN;
+      st->mu = muc;
+      st->NNN = NNN;
+    } else {
+      st->scale = 1.0;
+      muc = ((sB + sK) + f.g * sC) * w / ncur;
+    }
+    st->muc = muc;
+    st->ncur = ncur;
+  } else if (f.mode == FIN_SPINOR) {
+    // lib/@GPEtaskSpinor/solve_split.m:100-115: ncur, optional renormalisation, mu = <phi|H phi>/NNN
+    const double ncur = sA * w;
+    double NNN = st->NNN;
+    double q = 1.0;
+    if (f.renorm) {
+      if (f.tau > 0) NNN = f.NN0 * exp(-f.time2 / f.tau);
+      q = NNN / ncur;
+    }
+    st->scale = f.renorm ? sqrt(q) : 1.0;
+    const double mu = (q * ((sB + sK) + 2.0 * sx) + (q * q) * st->sC) * w / NNN;
+    st->mu = mu;
+    st->muc = mu;
+    st->ncur = ncur;
+    st->NNN = NNN;
+  } else if (f.mode == FIN_ITP) {
+    // groundstate_itp.m:44-52
+    double mu;
+    if (f.Ntotal > 0) {
+      mu = sqrt(f.Ntotal / (sA * w));
+      if (f.MU) f.MU[f.iter] = log(mu) / f.dt;
+    } else {
+      mu = exp(f.mu_init * f.dt);
+      if (f.MU) f.MU[f.iter] = (sA * w) * (mu * mu);
+    }
+    st->mu_itp = mu;
+  }
+}
+
+}  // namespace gpe

@@ -1,0 +1,170 @@
+// Body of the pencil sweeps, shared by the single-grid kernels
+// (sweep_kernels.hip) and the batched ensemble kernels (batch_kernels.hip).
+//
+// Work-group = 256 threads = C pencils x T threads; every thread holds E
+// elements of one pencil in registers (see fft_dev.h).  Grid indices are 32-bit
+// (the library limits a grid to 2^27 points), inactive tail pencils load a
+// valid pencil and skip the store, and the twiddle table is staged in LDS.
+#pragma once
+#include "fft_dev.h"
+#include "phase_dev.h"
+#include "reduce_dev.h"
+#include "sweep.h"
+
+namespace gpe {
+
+template <int SEQ> struct SeqTraits {
+  static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_KEX || SEQ == S_ITPB;
+  static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE);
+};
+
+// One work-group's share of a sweep: pencils tile*C .. tile*C + C-1.  `lds`
+// holds Lds<N, C, COLFAST>::SIZE doubles, `twl` N twiddles, `red` 4 doubles;
+// reductions go to partials[slot * maxwg + wg].
+template <int N, int C, bool COLFAST, int SEQ>
+__device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, unsigned wg, double* lds,
+                                           double2* twl, double* red) {
+  using PL = Plan<N>;
+  using TR = SeqTraits<SEQ>;
+  constexpr int T = PL::T, E = PL::E;
+  static_assert(C * T == 256, "work-group is 256 threads");
+
+  const int tid = threadIdx.x;
+  const int c = COLFAST ? tid % C : tid / T;
+  const int t = COLFAST ? tid / C : tid % T;
+  for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
+
+  const unsigned cgu = tile * (unsigned)C + (unsigned)c;
+  const bool active = cgu < p.ncols;
+  const unsigned cg = active ? cgu : p.ncols - 1u;
+  const unsigned base = (cg % p.A) * p.sa + (cg / p.A) * p.sb + (unsigned)t * p.stride;
+  const unsigned step = (unsigned)T * p.stride;
+
+  double2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = p.psi[base + (unsigned)m * step];
+  double vv[TR::LOAD_V ? E : 1];
+  if constexpr (TR::LOAD_V) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) vv[m] = p.V[base + (unsigned)m * step];
+  }
+
+  const bool has_re = p.dr != 0.0;
+  double accA = 0.0, accB = 0.0, accC = 0.0, accK = 0.0;
+
+  if constexpr (SEQ == S_OPEN) {
+    // solve_split.m:55-56 with the pending renormalisation (solve_split.m:72-73) folded in
+    const double s = p.st->scale, mu = p.st->mu;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double2 x = cscale(v[m], s);
+      const double a = -(vv[m] - mu) - p.g * cabs2(x);
+      v[m] = mul_cexp(x, a * p.dr * 0.5, a * p.di * 0.5, has_re);
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_MID || SEQ == S_CLOSE) {
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+    const double mu = p.st->mu;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double vm = vv[m] - mu;
+      const double a = -vm - p.g * cabs2(v[m]);                  // solve_split.m:60
+      double2 x = mul_cexp(v[m], a * p.dr, a * p.di, has_re);     // solve_split.m:61
+      if constexpr (SEQ == S_CLOSE) {
+        const double b = vm + p.g * cabs2(x);                      // solve_split.m:63
+        x = mul_cexp(x, b * p.dr * 0.5, b * p.di * 0.5, has_re);   // solve_split.m:64
+      }
+      v[m] = x;
+    }
+    if constexpr (SEQ == S_MID) fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_FWD) {
+    if (p.apply_scale) {
+      const double s = p.st->scale;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cscale(v[m], s);
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_INV) {
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_KIN) {
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    const double2 kc = p.kcol[cg];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, p.kax[t + T * m]), v[m]);
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_KEX || SEQ == S_KE) {
+    if constexpr (SEQ == S_KEX) {
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const double n = cabs2(v[m]);
+          accA += n;
+          accB += vv[m] * n;
+          accC += n * n;
+        }
+      }
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
+    }
+  } else if constexpr (SEQ == S_ITPA) {
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double nl = p.nop[base + (unsigned)m * step];
+      v[m] = cscale(v[m], nl);                                     // groundstate_itp.m:42
+      accA += active ? cabs2(v[m]) : 0.0;                          // groundstate_itp.m:44
+    }
+  } else if constexpr (SEQ == S_ITPB) {
+    const double mu = p.st->mu_itp;
+    const double mu2 = mu * mu;
+    const double hdt = -p.dr * 0.5;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double n = cabs2(v[m]);                                // nlinop before renormalisation
+      const double nl = exp_cw(hdt * (n * p.g * mu2 + vv[m]));     // groundstate_itp.m:76 (36: mu = 1)
+      if (active) p.nop[base + (unsigned)m * step] = nl;
+      v[m] = cscale(cscale(v[m], mu), nl);                         // groundstate_itp.m:52 then 40
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  }
+
+  if constexpr (TR::STORE) {
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+    }
+  }
+
+  // deterministic block reductions
+  if constexpr (SEQ == S_KEX) {
+    const double a = block_sum(accA, red);
+    const double b = block_sum(accB, red);
+    const double cc = block_sum(accC, red);
+    const double k = block_sum(accK, red);
+    if (tid == 0) {
+      p.partials[SL_A * p.maxwg + wg] = a;
+      p.partials[SL_B * p.maxwg + wg] = b;
+      p.partials[SL_C * p.maxwg + wg] = cc;
+      p.partials[p.slot_ke * p.maxwg + wg] = k;
+    }
+  } else if constexpr (SEQ == S_KE) {
+    const double k = block_sum(accK, red);
+    if (tid == 0) p.partials[p.slot_ke * p.maxwg + wg] = k;
+  } else if constexpr (SEQ == S_ITPA) {
+    const double a = block_sum(accA, red);
+    if (tid == 0) p.partials[SL_A * p.maxwg + wg] = a;
+  }
+}
+
+template <int N, int C, bool COLFAST, int SEQ>
+__global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
+  __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
+  __shared__ double2 twl[N];
+  __shared__ double red[4];
+  sweep_body<N, C, COLFAST, SEQ>(p, blockIdx.x, blockIdx.x, lds, twl, red);
+}
+
+}  // namespace gpe

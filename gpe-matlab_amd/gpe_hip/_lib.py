@@ -63,6 +63,12 @@ SIGNATURES = {
     "gpe_control_set": [_P, C.c_int, _DP, _DP],
     "gpe_control_forward": [_P, C.POINTER(ControlArgs), _DP, _DP],
     "gpe_adjoint_grad": [_P, C.POINTER(ControlArgs), _DP, _DP, _DP, _DP, _DP],
+    "gpe_batch_create": [C.c_int, C.POINTER(GridDesc), C.POINTER(C.c_void_p)],
+    "gpe_batch_destroy": [_P],
+    "gpe_batch_set": [_P, C.c_int, C.POINTER(Params), _DP, _DP],
+    "gpe_batch_get_state": [_P, C.c_int, _DP],
+    "gpe_batch_split_begin": [_P, _D, C.c_int, C.c_int, _DP],
+    "gpe_batch_split_outer": [_P, C.c_int, _DP, _DP, _DP],
 }
 OPTIONAL = set()
 
@@ -90,6 +96,8 @@ def load():
         f.restype = C.c_int
     lib.gpe_last_error.argtypes = [_P]
     lib.gpe_last_error.restype = C.c_char_p
+    lib.gpe_batch_last_error.argtypes = [_P]
+    lib.gpe_batch_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -263,6 +271,83 @@ class Device:
 
 SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
+
+
+def make_params(ncomp, g, gamma, decay_rate, Ntotal, mu_init, n_recalc, nlincpl=0.0):
+    p = Params()
+    p.ncomp = ncomp
+    gm = np.asarray(g, dtype=np.float64)
+    if ncomp == 1:
+        p.g[0] = float(gm.ravel()[0])
+    else:
+        flat = gm.reshape(2, 2).T.ravel()
+        for i in range(4):
+            p.g[i] = float(flat[i])
+    p.gamma, p.decay_rate, p.Ntotal, p.mu_init = float(gamma), float(decay_rate), float(Ntotal), float(mu_init)
+    p.n_recalc = int(n_recalc)
+    p.nlincpl = float(nlincpl)
+    return p
+
+
+class Batch:
+    """Owns one gpe_batch (ensemble of 2D grids)."""
+
+    def __init__(self, grids):
+        self.lib = load()
+        n = len(grids)
+        arr = (GridDesc * n)()
+        for i, g in enumerate(grids):
+            if g.ndims != 2:
+                raise GPELibraryError("the ensemble path takes 2D grids")
+            arr[i].ndims = 2
+            arr[i].n[0], arr[i].n[1], arr[i].n[2] = g.nx, g.ny, 1
+            arr[i].xmax[0], arr[i].xmax[1], arr[i].xmax[2] = g.xmax, g.ymax, 1.0
+        h = C.c_void_p()
+        rc = self.lib.gpe_batch_create(n, arr, C.byref(h))
+        if rc != GPE_OK:
+            raise GPELibraryError(f"gpe_batch_create failed ({ERRNAMES.get(rc, rc)})")
+        self.h = h
+        self.shapes = [g.size for g in grids]
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gpe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != GPE_OK:
+            m = self.lib.gpe_batch_last_error(self.h)
+            raise GPELibraryError(f"{name}: {ERRNAMES.get(rc, rc)} {m.decode() if m else ''}")
+
+    def set(self, i, params, V, psi):
+        v = to_colmajor_f64(np.broadcast_to(V, self.shapes[i]))
+        s = to_colmajor_c128(np.broadcast_to(psi, self.shapes[i]))
+        self.call("gpe_batch_set", int(i), C.byref(params), dptr(v), s.view(np.float64).ctypes.data_as(_DP))
+
+    def get_state(self, i):
+        out = np.empty(int(np.prod(self.shapes[i])), dtype=np.complex128)
+        self.call("gpe_batch_get_state", int(i), out.view(np.float64).ctypes.data_as(_DP))
+        return from_colmajor_c128(out, self.shapes[i])
+
+    def split_begin(self, ddt, niter_inner, use_graph=True):
+        muc = np.zeros(self.n)
+        self.call("gpe_batch_split_begin", float(ddt), int(niter_inner), int(bool(use_graph)), dptr(muc))
+        return muc
+
+    def split_outer(self, j):
+        t = C.c_double()
+        muc = np.zeros(self.n)
+        ncur = np.zeros(self.n)
+        self.call("gpe_batch_split_outer", int(j), C.byref(t), dptr(muc), dptr(ncur))
+        return t.value, muc, ncur
 
 
 def selftest_phase(er, ei):

@@ -52,3 +52,30 @@ def c2_state(n=256, xmax=12.0, seed=1):
     theta = rng.uniform(0.0, 0.1, grid.size)
     psi0 = tf * np.exp(1j * theta)
     return grid, V, psi0, props
+
+
+def c4_ensemble(ntraj=256, sizes=(128, 192, 256, 384), xmax=15.0, Ntotal=1e4, seed=4):
+    """Ragged ensemble: grid sizes uniform over `sizes` (square), g ~ U[0.01, 0.1],
+    V = (X^2 + lam^2 Y^2)/2 with lam ~ U[1, 2], Thomas-Fermi state with a vortex
+    imprinted at a seeded position |r0| < 5.  Returns a list of
+    (grid, V, psi0, props) tuples."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(ntraj):
+        n = int(rng.choice(sizes))
+        g = float(rng.uniform(0.01, 0.1))
+        lam = float(rng.uniform(1.0, 2.0))
+        r0 = 5.0 * np.sqrt(rng.uniform()) * np.exp(2j * np.pi * rng.uniform())
+        grid = grid2d(xmax, n, xmax, n)
+        V = 0.5 * (grid.mesh.x ** 2 + lam ** 2 * grid.mesh.y ** 2)
+        tf, _ = thomas_fermi(grid, V, Ntotal, g)
+        psi = imprint_vortex_at(grid, tf.astype(np.complex128), g, r0.real, r0.imag)
+        out.append((grid, V, psi, dict(Ntotal=Ntotal, g=g)))
+    return out
+
+
+def imprint_vortex_at(grid, phi, g, x0, y0):
+    """phi * tanh(r/xi) * exp(i atan2(x-x0, y-y0)) (examples/example1_vortex.m:26-30 form)."""
+    xi = 1 / np.sqrt(2 * g * np.max(np.abs(phi)) ** 2)
+    rr = np.sqrt((grid.mesh.x - x0) ** 2 + (grid.mesh.y - y0) ** 2)
+    return phi * np.tanh(rr / xi) * np.exp(1j * np.arctan2(grid.mesh.x - x0, grid.mesh.y - y0))

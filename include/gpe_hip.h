@@ -137,6 +137,25 @@ int gpe_control_forward(gpe_handle* h, const gpe_control_args* a, const double* 
 int gpe_adjoint_grad(gpe_handle* h, const gpe_control_args* a, const double* coeffs, const double* psi_target,
                      double* loss, double* grad, double* recon_err);
 
+/* ------------------------------------------------------------------ ensemble
+ * (no reference counterpart: north-star form 2).  A batch advances many
+ * independent 2D GPEtask.solve_split runs together; each trajectory has its own
+ * grid (lengths may differ), potential, state and gpe_params (g, gamma, Ntotal > 0,
+ * mu_init, decay_rate; one common n_recalc).  Every sweep is one launch per
+ * distinct pencil length for the whole ensemble; with use_graph and no
+ * decay_rate, a whole outer iteration is replayed from one captured hipGraph.
+ * Trajectory i follows solve_split.m:13-84 exactly as a single handle would. */
+typedef struct gpe_batch gpe_batch;
+int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out);
+int gpe_batch_destroy(gpe_batch* b);
+const char* gpe_batch_last_error(const gpe_batch* b);
+int gpe_batch_set(gpe_batch* b, int i, const gpe_params* p, const double* V, const double* psi);
+int gpe_batch_get_state(gpe_batch* b, int i, double* psi);
+/* muc_out[ntraj] receives each trajectory's initial muc (solve_split.m:41). */
+int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_graph, double* muc_out);
+/* One outer iteration j for every trajectory; muc[ntraj], ncur[ntraj] as passed to ext_callback. */
+int gpe_batch_split_outer(gpe_batch* b, int j, double* time2, double* muc, double* ncur);
+
 /* Measurement hooks (no reference counterpart): time every sweep launch with
  * HIP events recorded on the handle's own stream.  gpe_profile_read fills
  * ms[seq*3 + axis] (summed milliseconds) and count[seq*3 + axis] for
