@@ -413,7 +413,9 @@ int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_gra
   BCHK(b, hipMemcpy(s.data(), b->st, ntraj * sizeof(DevState), hipMemcpyDeviceToHost));
   for (int i = 0; i < ntraj; ++i) {
     const auto& t = b->tr[i];
-    const double muc = ((s[i].sB + s[i].sK) + t.prm.g[0] * s[i].sC) * t.w / t.prm.Ntotal;
+    // same association as gpe_split_begin: ((sum V n w + kinetic w) + g sum n^2 w) / NN0
+    const double m1 = s[i].sB * t.w, m2 = s[i].sC * t.w, m3 = s[i].sK * t.w;
+    const double muc = ((m1 + m3) + t.prm.g[0] * m2) / t.prm.Ntotal;
     s[i].mu = t.prm.mu_init > 0 ? t.prm.mu_init : muc;
     s[i].scale = 1.0;
     s[i].NNN = t.prm.Ntotal;
