@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
+                         "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
 
     if args.copy_calib:
@@ -90,6 +93,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
               "the multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "c2":
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+        return
 
     import gpe_hip
     from gpe_hip import workloads
@@ -205,6 +211,229 @@ def main():
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
     }
     print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, steps, warmup, el, hib, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(1e3 * el / max(steps, 1), 4), "higher_is_better": hib, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_c1(args):
+    """2D GPEtask: groundstate_itp(0.02, 1e-6) from a seeded random guess, vortex at x0 = 5,
+    solve_split(0.004, 50, 500) -- end to end on the GPU and in the oracle."""
+    import gpe_hip
+    from gpe_hip import workloads
+    cfg = workloads.c1_vortex_2d()
+    G = gpe_hip.grid2d(*cfg["grid"][1])
+    t = gpe_hip.GPEtask(G, cfg["V"], seed=cfg["seed"])
+    t.Ntotal, t.g = cfg["props"]["Ntotal"], cfg["props"]["g"]
+    t0 = time.perf_counter()
+    phi, MU = t.groundstate_itp(*cfg["itp"], nargout=2)
+    t_itp = time.perf_counter() - t0
+    t.init_state = workloads.imprint_vortex(G, phi, t.g)
+    ddt, ni, no = cfg["split"]   # the full configuration: 500 outer x 50 inner steps
+    sync()
+    t0 = time.perf_counter()
+    out = t.solve_split(ddt, ni, no)
+    sync()
+    el = time.perf_counter() - t0
+    steps = ni * no
+    log(f"c1 GPU: ITP {t.itp_iterations} iterations in {t_itp:.2f}s; {steps} split steps in {el:.2f}s")
+    parity = cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        O = oracle.grid2d(*cfg["grid"][1])
+        to = oracle.GPEtask(O, cfg["V"], seed=cfg["seed"])
+        to.Ntotal, to.g = t.Ntotal, t.g
+        c0 = time.perf_counter()
+        po, MUo = to.groundstate_itp(*cfg["itp"], nargout=2)
+        c_itp = time.perf_counter() - c0
+        to.init_state = workloads.imprint_vortex(O, po, to.g)
+        c0 = time.perf_counter()
+        pso = to.solve_split(ddt, ni, no)
+        c_el = time.perf_counter() - c0
+        parity = {"itp_iterations_gpu": t.itp_iterations, "itp_iterations_oracle": to.itp_iterations,
+                  "itp_state_rel_l2": float(np.linalg.norm(phi - po) / np.linalg.norm(po)),
+                  "final_state_rel_l2": float(np.linalg.norm(out - pso) / np.linalg.norm(pso)),
+                  "history_mu_max_rel": float(np.max(np.abs(t.history["mu"] / to.history["mu"] - 1)))}
+        from oracle import grids as ogrids
+        cpu = {"value": round(steps / c_el, 2), "unit": "split-steps/s", "cores": ogrids.WORKERS, "kind": "port",
+               "sample": f"oracle: ITP {to.itp_iterations} it in {c_itp:.1f}s, {steps} split steps in {c_el:.1f}s"}
+        log(f"c1 oracle: ITP {to.itp_iterations} it {c_itp:.1f}s, split {c_el:.1f}s; parity {parity}")
+    _line("split-steps/s (GPEtask 2D 256^2 vortex dynamics)", round(steps / el, 1), "split-steps/s", steps, 0, el,
+          True, {"workload": "c1: grid2d(17,256,17,256) V=0.5(X^2+Y^2) Ntotal=1e5 g=0.06; groundstate_itp(0.02,1e-6) "
+                             f"+ vortex x0=5 + solve_split(0.004,{ni},{no})", "grid": [256, 256]},
+          itp_iterations=t.itp_iterations, itp_seconds=round(t_itp, 3), parity_vs_oracle=parity, cpu_baseline=cpu)
+
+
+def bench_c3(args):
+    """Adjoint gradient of 1 - |<psi_t|psi(T)>|^2 w.r.t. 64 control amplitudes, 512^2, T = 2000."""
+    import gpe_hip
+    from gpe_hip.adjoint import SplitStepControl
+    rng = np.random.Generator(np.random.PCG64(2))
+    G = gpe_hip.grid2d(20, 512, 20, 512)
+    V0 = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2)
+    ab = rng.uniform(-8, 8, (8, 2))
+    basis = [np.exp(-((G.mesh.x - a) ** 2 + (G.mesh.y - b) ** 2) / 2) for a, b in ab]
+    c = rng.normal(0, 0.5, (8, 8))
+    ct = np.random.Generator(np.random.PCG64(3)).normal(0, 0.5, (8, 8))
+    psi0 = G.normalize(np.exp(-((G.mesh.x - 1) ** 2 + G.mesh.y ** 2) / 2) + 0j)
+    t = gpe_hip.GPEtask(G, V0)
+    t.g, t.Ntotal = 10.0, 1.0
+    t.init_state = psi0
+    T, ddt, spi = 2000, 0.002, 250
+    ctl = SplitStepControl(t, V0, basis, ddt, T, spi)
+    psi_t = ctl.forward(ct)
+    ctl.forward(c)
+    sync()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctl.forward(c)
+    sync()
+    t_fwd = (time.perf_counter() - t0) / reps
+    ctl.loss_and_grad(c, psi_t)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L, g, rerr = ctl.loss_and_grad(c, psi_t)
+    sync()
+    t_grad = (time.perf_counter() - t0) / reps
+    # plain solve_split forward of the same length and potential schedule (Vtd per block)
+    ratio = t_grad / t_fwd
+    h = 1e-6
+    fd = []
+    ad = []
+    for (k, m) in [(0, 0), (3, 2), (5, 5), (7, 7), (1, 4), (6, 1)]:
+        cp_, cm_ = c.copy(), c.copy()
+        cp_[k, m] += h
+        cm_[k, m] -= h
+        Lp = 1 - abs(G.inner(psi_t, ctl.forward(cp_))) ** 2
+        Lm = 1 - abs(G.inner(psi_t, ctl.forward(cm_))) ** 2
+        fd.append((Lp - Lm) / (2 * h))
+        ad.append(g[k, m])
+    fd, ad = np.array(fd), np.array(ad)
+    fd_err = float(np.linalg.norm(ad - fd) / np.linalg.norm(fd))
+    log(f"c3: forward {t_fwd * 1e3:.1f} ms, gradient {t_grad * 1e3:.1f} ms, ratio {ratio:.2f}; loss {L:.6f}; "
+        f"grad vs FD rel {fd_err:.2e}; reconstruction {rerr:.1e}")
+    _line("adjoint/forward cost ratio (512^2, T=2000, 64 controls)", round(ratio, 3), "x forward", T, 0, t_grad,
+          False, {"workload": "c3: grid2d(20,512,20,512) V=0.5(X^2+Y^2)+sum_k c_k(t) exp(-((X-a_k)^2+(Y-b_k)^2)/2), "
+                              "8 bumps x 8 intervals of 250 steps, dt=0.002, T=2000, g=10, Ntotal=1, gamma=0",
+                  "grid": [512, 512]},
+          forward_ms=round(t_fwd * 1e3, 2), gradient_ms=round(t_grad * 1e3, 2), loss=L, grad_vs_fd_rel=fd_err,
+          fd_components=len(fd), backward_reconstruction_rel_l2=rerr)
+
+
+def bench_c4(args):
+    """256 ragged 2D trajectories: batched launches + graph vs one handle per trajectory."""
+    import gpe_hip
+    from gpe_hip import _lib, workloads
+    ens = workloads.c4_ensemble()
+    grids = [e[0] for e in ens]
+    npts = sum(int(np.prod(gr.size)) for gr in grids)
+    ddt, ni = 0.004, 100
+    b = _lib.Batch(grids)
+    for i, (gr, V, psi, pr) in enumerate(ens):
+        b.set(i, _lib.make_params(1, pr["g"], 0.0, 0.0, pr["Ntotal"], 0.0, 10), V, psi)
+    b.split_begin(ddt, ni, True)
+    b.split_outer(1)
+    sync()
+    nout = max(1, args.steps // ni)
+    t0 = time.perf_counter()
+    for j in range(2, 2 + nout):
+        b.split_outer(j)
+    sync()
+    el = time.perf_counter() - t0
+    ens_steps = nout * ni
+    traj_rate = len(ens) * ens_steps / el
+    # per-trajectory launches: one handle per trajectory, the same steps
+    devs = []
+    for gr, V, psi, pr in ens:
+        d = _lib.Device(gr)
+        d.set_params(1, pr["g"], 0.0, 0.0, pr["Ntotal"], 0.0, 10)
+        d.set_potential(0, V)
+        d.set_state(0, psi)
+        d.split_begin(ddt, ni)
+        devs.append(d)
+    for d in devs:
+        d.split_outer(1)
+    sync()
+    t0 = time.perf_counter()
+    for d in devs:
+        for j in range(2, 2 + nout):
+            d.split_outer(j)
+    sync()
+    el1 = time.perf_counter() - t0
+    single_rate = len(ens) * ens_steps / el1
+    same = all(np.array_equal(b.get_state(i), devs[i].get_state(0)) for i in range(0, len(ens), 37))
+    gbps = 72 * npts * (ens_steps / el) / 1e9
+    log(f"c4: {len(ens)} trajectories, {npts} points; batched {traj_rate:.0f} traj-steps/s, per-trajectory "
+        f"{single_rate:.0f} traj-steps/s, ratio {traj_rate / single_rate:.2f}; bit-identical sample {same}")
+    _line("trajectory-steps/s (256 ragged 2D trajectories)", round(traj_rate, 1), "trajectory-steps/s", ens_steps, ni,
+          el, True, {"workload": "c4: 256 x grid2d(15,N,15,N), N in {128,192,256,384}, g~U[0.01,0.1], "
+                                 "V=0.5(X^2+lam^2 Y^2) lam~U[1,2], TF + vortex |r0|<5, dt=0.004, graph per 100 steps",
+                     "points": npts},
+          per_trajectory_launch_rate=round(single_rate, 1), batched_vs_per_trajectory=round(traj_rate / single_rate, 2),
+          ensemble_alg_GBps=round(gbps, 1), batched_equals_single_bitwise=bool(same))
+
+
+def bench_c5(args):
+    """GPEtaskSpinor 3D 192^3: fused Rabi step, Omega(x) = 0.5 exp(2iX)."""
+    import gpe_hip
+    from gpe_hip import workloads
+    n = 192
+    G = gpe_hip.grid3d(12, n, 12, n, 12, n)
+    V = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2 + G.mesh.z ** 2)
+    tf, _ = workloads.thomas_fermi(G, V, 1e6, 0.02)
+    rng = np.random.Generator(np.random.PCG64(5))
+    a = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    bcomp = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    gm = np.array([[0.02, 0.018], [0.018, 0.02]])
+    om = 0.5 * np.exp(2j * G.mesh.x)
+    t = gpe_hip.GPEtaskSpinor(G, [V, V])
+    t.g, t.coupling, t.Ntotal = gm, om, 1e6
+    t.init_state = [a, bcomp]
+    dev = t._device()
+    t._set_params(1e6)
+    t._upload([a, bcomp])
+    t._upload_potential([V, V])
+    t._upload_coupling()
+    dev.split_begin(0.002, 10)
+    parity = None
+    if not args.no_parity:
+        dev.split_outer(1)
+        g10 = [dev.get_state(0), dev.get_state(1)]
+        import oracle
+        O = oracle.grid3d(12, n, 12, n, 12, n)
+        to = oracle.GPEtaskSpinor(O, [V, V])
+        to.g, to.coupling, to.Ntotal = gm, om, 1e6
+        to.init_state = [a, bcomp]
+        c0 = time.perf_counter()
+        po = to.solve_split(0.002, 10, 1)
+        cel = time.perf_counter() - c0
+        parity = max(float(np.linalg.norm(g10[i] - po[i]) / np.linalg.norm(po[i])) for i in range(2))
+        log(f"c5 parity (10 steps) rel-L2 {parity:.2e}; oracle {cel:.1f}s")
+        t._upload([a, bcomp])
+        dev.split_begin(0.002, 10)
+    dev.split_outer(1)
+    sync()
+    nout = max(1, args.steps // 10)
+    t0 = time.perf_counter()
+    for j in range(2, 2 + nout):
+        dev.split_outer(j)
+    sync()
+    el = time.perf_counter() - t0
+    steps = nout * 10
+    rate = steps / el
+    gbps = 288 * n ** 3 * rate / 1e9
+    log(f"c5: {steps} spinor steps in {el:.3f}s -> {rate:.1f} steps/s, {gbps:.0f} GB/s algorithmic")
+    _line("spinor split-steps/s (GPEtaskSpinor 3D 192^3, 2 components)", round(rate, 2), "split-steps/s", steps, 10, el,
+          True, {"workload": "c5: grid3d(12,192,...) V1=V2=0.5(X^2+Y^2+Z^2), g=[0.02 0.018;0.018 0.02], "
+                             "Omega=0.5 exp(2iX), Ntotal=1e6, TF split 50/50, dt=0.002, n_recalc=10", "grid": [n] * 3},
+          step_alg_GBps=round(gbps, 1), step_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
+          parity_rel_l2_vs_oracle_10_steps=parity)
 
 
 if __name__ == "__main__":
