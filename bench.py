@@ -85,7 +85,13 @@ def main():
         for _ in range(5):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes_read": x.numel() * 16, "copy_calib_bytes_written": x.numel() * 16}))
+        t0 = time.perf_counter()
+        for _ in range(50):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 50
+        print(json.dumps({"copy_calib_bytes_read": x.numel() * 16, "copy_calib_bytes_written": x.numel() * 16,
+                          "copy_GBps_read_plus_write": round(2 * x.numel() * 16 / el / 1e9, 1)}))
         return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

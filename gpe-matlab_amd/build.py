@@ -36,15 +36,23 @@ def flags():
             "-I", os.path.join(HERE, "..", "include"), "-I", CSRC]
 
 
+DEFINES = []
+
+
 def compile_one(src, obj, extra=()):
-    cmd = [hipcc(), *flags(), *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *flags(), *DEFINES, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(jobs=None, verbose=True):
+def build(jobs=None, verbose=True, out=None, build_dir=None):
+    global BUILD, OUT
+    if out:
+        OUT = out
+    if build_dir:
+        BUILD = build_dir
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     tasks = []
@@ -75,9 +83,13 @@ def build(jobs=None, verbose=True):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int, default=None)
+    ap.add_argument("-D", "--define", action="append", default=[], help="extra -D for every source")
+    ap.add_argument("--out", default=None, help="output library path (default gpe_hip/libgpehip.so)")
+    ap.add_argument("--build-dir", default=None)
     a = ap.parse_args()
+    DEFINES.extend(f"-D{d}" for d in a.define)
     try:
-        build(a.jobs)
+        build(a.jobs, out=a.out, build_dir=a.build_dir)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -28,18 +28,26 @@ __device__ __forceinline__ void finalize_core(const FinParams& f, double* red) {
     if (threadIdx.x == 0) st->delta = m / (f.dt * 10.0);   // groundstate_itp.m:61
     return;
   }
-  const double sA = f.cnt[SL_A] ? slot_sum(f.partials, f.maxwg, SL_A, f.cnt[SL_A], red) : 0.0;
-  const double sB = f.cnt[SL_B] ? slot_sum(f.partials, f.maxwg, SL_B, f.cnt[SL_B], red) : 0.0;
-  const double sC = f.cnt[SL_C] ? slot_sum(f.partials, f.maxwg, SL_C, f.cnt[SL_C], red) : 0.0;
-  const double kx = f.cnt[SL_KX] ? slot_sum(f.partials, f.maxwg, SL_KX, f.cnt[SL_KX], red) : 0.0;
-  const double ky = f.cnt[SL_KY] ? slot_sum(f.partials, f.maxwg, SL_KY, f.cnt[SL_KY], red) : 0.0;
-  const double kz = f.cnt[SL_KZ] ? slot_sum(f.partials, f.maxwg, SL_KZ, f.cnt[SL_KZ], red) : 0.0;
-  const double c12 = f.cnt[SL_C12] ? slot_sum(f.partials, f.maxwg, SL_C12, f.cnt[SL_C12], red) : 0.0;
-  const double c2 = f.cnt[SL_C2] ? slot_sum(f.partials, f.maxwg, SL_C2, f.cnt[SL_C2], red) : 0.0;
-  const double sx = f.cnt[SL_X] ? slot_sum(f.partials, f.maxwg, SL_X, f.cnt[SL_X], red) : 0.0;
-  const double kx2 = f.cnt[SL_KX2] ? slot_sum(f.partials, f.maxwg, SL_KX2, f.cnt[SL_KX2], red) : 0.0;
-  const double ky2 = f.cnt[SL_KY2] ? slot_sum(f.partials, f.maxwg, SL_KY2, f.cnt[SL_KY2], red) : 0.0;
-  const double kz2 = f.cnt[SL_KZ2] ? slot_sum(f.partials, f.maxwg, SL_KZ2, f.cnt[SL_KZ2], red) : 0.0;
+  // one pass over the partials with every used slot's loads in flight together
+  constexpr int NS = 12;
+  const int slots[NS] = {SL_A, SL_B, SL_C, SL_KX, SL_KY, SL_KZ, SL_C12, SL_C2, SL_X, SL_KX2, SL_KY2, SL_KZ2};
+  double acc[NS];
+  int maxc = 0;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    acc[q] = 0.0;
+    maxc = f.cnt[slots[q]] > maxc ? f.cnt[slots[q]] : maxc;
+  }
+  for (int i = threadIdx.x; i < maxc; i += 256) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (i < f.cnt[slots[q]]) acc[q] += f.partials[slots[q] * f.maxwg + i];
+  }
+  double tot[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) tot[q] = f.cnt[slots[q]] ? block_sum(acc[q], red) : 0.0;
+  const double sA = tot[0], sB = tot[1], sC = tot[2], kx = tot[3], ky = tot[4], kz = tot[5];
+  const double c12 = tot[6], c2 = tot[7], sx = tot[8], kx2 = tot[9], ky2 = tot[10], kz2 = tot[11];
   if (threadIdx.x != 0) return;
   const double sK = ((kx + ky) + kz) + ((kx2 + ky2) + kz2);
   st->sA = sA; st->sB = sB; st->sK = sK;

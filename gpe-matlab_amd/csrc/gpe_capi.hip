@@ -43,6 +43,8 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     delete h;
     return GPE_EHIP;
   }
+  const char* snake = getenv("GPE_SNAKE");
+  h->snake = snake ? atoi(snake) : 0;
   h->ndims = d->ndims;
   h->nx = d->n[0];
   h->ny = d->n[1];

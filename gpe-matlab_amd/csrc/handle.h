@@ -79,6 +79,7 @@ struct gpe_handle {
   int adj_gradcap = 0;
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
+  int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -297,15 +298,19 @@ inline int run_block(gpe_handle* h, double time2) {
   const double* V = h->V[0];
   for (int i = 0; i < h->n_rec; ++i) {
     const int xseq = i == 0 ? gpe::S_OPEN : gpe::S_MID;
-    if (i > 0) {
-      if ((rc = sweep(h, 0, gpe::S_MID, base_params(h, 0, psi, V)))) return rc;
-    } else {
-      if ((rc = sweep(h, 0, xseq, base_params(h, 0, psi, V)))) return rc;
-    }
+    SweepParams px = base_params(h, 0, psi, V);
+    px.reverse = (h->snake & 4) ? 1 : 0;
+    if ((rc = sweep(h, 0, i > 0 ? gpe::S_MID : xseq, px))) return rc;
     if (h->ndims == 3) {
-      if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, V)))) return rc;
-      if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
-      if ((rc = sweep(h, 1, gpe::S_INV, base_params(h, 1, psi, V)))) return rc;
+      SweepParams py = base_params(h, 1, psi, V);
+      py.reverse = (h->snake & 1) ? 1 : 0;
+      if ((rc = sweep(h, 1, gpe::S_FWD, py))) return rc;
+      SweepParams pz = base_params(h, 2, psi, V);
+      pz.reverse = (h->snake & 8) ? 1 : 0;
+      if ((rc = sweep(h, 2, gpe::S_KIN, pz))) return rc;
+      SweepParams pi = base_params(h, 1, psi, V);
+      pi.reverse = (h->snake & 2) ? 1 : 0;
+      if ((rc = sweep(h, 1, gpe::S_INV, pi))) return rc;
     } else {
       if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
     }

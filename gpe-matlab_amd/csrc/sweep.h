@@ -64,6 +64,7 @@ struct SweepParams {
   double g;                  // interaction constant
   double dr, di;             // complex time step dt = dr + i di (ITP: dr = dt, di = 0)
   int apply_scale;           // S_FWD: multiply by the pending scale st->scale first
+  int reverse;               // visit pencil tiles in reverse order (cache reuse between sweeps)
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

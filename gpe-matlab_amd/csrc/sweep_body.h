@@ -159,12 +159,24 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   }
 }
 
+// Minimum waves per SIMD requested from the register allocator for the kinetic
+// and the phase sweeps (1 = no constraint).
+#ifndef GPE_WAVES_KIN
+#define GPE_WAVES_KIN 1
+#endif
+#ifndef GPE_WAVES_PHASE
+#define GPE_WAVES_PHASE 1
+#endif
+
 template <int N, int C, bool COLFAST, int SEQ>
-__global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    SEQ == S_KIN ? GPE_WAVES_KIN : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE) ? GPE_WAVES_PHASE : 1)))
+    sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[N];
   __shared__ double red[4];
-  sweep_body<N, C, COLFAST, SEQ>(p, blockIdx.x, blockIdx.x, lds, twl, red);
+  const unsigned tile = p.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+  sweep_body<N, C, COLFAST, SEQ>(p, tile, tile, lds, twl, red);
 }
 
 }  // namespace gpe
