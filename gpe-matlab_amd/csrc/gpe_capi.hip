@@ -45,6 +45,8 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   }
   const char* snake = getenv("GPE_SNAKE");
   h->snake = snake ? atoi(snake) : 0;
+  const char* slabs = getenv("GPE_SLABS");
+  h->slabs = slabs ? atoi(slabs) : 0;
   h->ndims = d->ndims;
   h->nx = d->n[0];
   h->ny = d->n[1];

@@ -80,6 +80,7 @@ struct gpe_handle {
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
+  int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -292,7 +293,48 @@ inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
 }
 
 // One block of solve_split.m:53-78 (GPEtask, static or pre-uploaded V).
+// Sweep parameters restricted to the z planes [z0, z0 + nzs) of a 3D grid (x or y axis).
+inline SweepParams slab_params(gpe_handle* h, int axis, double2* psi, const double* V, int z0, int nzs) {
+  SweepParams p = base_params(h, axis, psi, V);
+  const size_t off = (size_t)z0 * (size_t)h->nx * (size_t)h->ny;
+  p.psi = psi + off;
+  p.V = V ? V + off : nullptr;
+  p.ncols = (unsigned)((axis == 0 ? h->ny : h->nx) * nzs);
+  return p;
+}
+
+// Block schedule with the plane-local sweeps of one step (y inverse, x phase, y
+// forward) run slab by slab, so a slab stays cache-resident across them; the
+// z kinetic sweep covers the whole grid.  Same arithmetic as run_block.
+inline int run_block_slabs(gpe_handle* h, double time2) {
+  int rc;
+  double2* psi = h->psi[0];
+  const double* V = h->V[0];
+  const int S = h->slabs, nzs = h->nz / h->slabs;
+  for (int i = 0; i < h->n_rec; ++i) {
+    for (int s = 0; s < S; ++s) {
+      if (i > 0 && (rc = sweep(h, 1, gpe::S_INV, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
+      if ((rc = sweep(h, 0, i == 0 ? gpe::S_OPEN : gpe::S_MID, slab_params(h, 0, psi, V, s * nzs, nzs)))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_FWD, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
+    }
+    if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
+  }
+  for (int s = 0; s < S; ++s) {
+    if ((rc = sweep(h, 1, gpe::S_INV, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
+    if ((rc = sweep(h, 0, gpe::S_CLOSE, slab_params(h, 0, psi, V, s * nzs, nzs)))) return rc;
+  }
+  FinParams f = fin_params(h, gpe::FIN_BLOCK);
+  if ((rc = launch_measure(h, 0, &f))) return rc;
+  f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
+  f.NN0 = h->NN0;
+  f.tau = h->prm.decay_rate;
+  f.time2 = time2;
+  HIPCHK(h, gpe::launch_finalize(f, h->stream));
+  return GPE_OK;
+}
+
 inline int run_block(gpe_handle* h, double time2) {
+  if (h->ndims == 3 && h->slabs > 1 && h->nz % h->slabs == 0) return run_block_slabs(h, time2);
   int rc;
   double2* psi = h->psi[0];
   const double* V = h->V[0];
