@@ -34,7 +34,7 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBPS = 8000.0
 # algorithmic HBM bytes per grid point per launch of each sweep sequence
 SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40, "kex": 24, "ke": 16,
-             "itpa": 40, "itpb": 48}
+             "itpa": 40, "itpb": 48, "closem": 40}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 
 
@@ -122,33 +122,47 @@ def main():
     npts = int(np.prod(grid.size))
 
     # ---------------------------------------------------------------- parity leg
+    # GPU and oracle run the same cs steps (outer iterations of 10 steps, one call),
+    # compared every 100 steps (or at the end when cs < 100).
     parity = None
-    gpu10 = None
-    cs = args.cpu_steps
-    if not args.no_parity or not args.no_cpu_baseline:
-        dev.set_state(0, psi0)
-        dev.split_begin(ddt, cs)
-        dev.split_outer(1)
-        gpu10 = dev.get_state(0)
-        log(f"parity leg: {cs} GPU steps done")
-
+    parity_curve = []
     cpu_baseline = None
-    if not args.no_cpu_baseline or not args.no_parity:
+    cs = args.cpu_steps
+    if not (args.no_parity and args.no_cpu_baseline):
+        every = 10 if cs < 100 else 100
+        dev.set_state(0, psi0)
+        dev.split_begin(ddt, 10)
+        gpu_ck = {}
+        for j in range(1, cs // 10 + 1):
+            dev.split_outer(j)
+            if (j * 10) % every == 0:
+                gpu_ck[j * 10] = dev.get_state(0)
+        log(f"parity leg: {cs} GPU steps done")
         import oracle
         from oracle import grids as ogrids
         O = oracle.grid3d(grid.xmax, grid.nx, grid.ymax, grid.ny, grid.zmax, grid.nz)
         to = oracle.GPEtask(O, V)
         to.Ntotal, to.g, to.gamma = props["Ntotal"], props["g"], props["gamma"]
         to.init_state = psi0
+        cpu_ck = {}
+
+        def grab(t):
+            if (t.current_iter * 10) % every == 0:
+                cpu_ck[t.current_iter * 10] = t.current_state.copy()
+
+        to.user_callback = grab
         t0 = time.perf_counter()
-        po = to.solve_split(ddt, cs, 1)
+        to.solve_split(ddt, 10, cs // 10)
         el = time.perf_counter() - t0
         cpu_baseline = {"value": cs / el, "unit": "split-steps/s", "cores": ogrids.WORKERS, "kind": "port",
-                        "sample": f"FP64 numpy/scipy restatement of solve_split.m, {cs} steps (one n_rec block) "
-                                  f"of the same 256^3 workload in {el:.1f}s; scipy.fft workers={ogrids.WORKERS}, "
+                        "sample": f"FP64 numpy/scipy restatement of solve_split.m, {cs} steps of the same 256^3 "
+                                  f"workload in {el:.1f}s; scipy.fft workers={ogrids.WORKERS}, "
                                   "elementwise passes single-threaded"}
-        parity = float(np.linalg.norm(gpu10 - po) / np.linalg.norm(po))
-        log(f"cpu leg: {cs} oracle steps in {el:.1f}s; GPU rel-L2 vs oracle {parity:.3e}")
+        for k in sorted(gpu_ck):
+            e = float(np.linalg.norm(gpu_ck[k] - cpu_ck[k]) / np.linalg.norm(cpu_ck[k]))
+            parity_curve.append([k, e])
+        parity = parity_curve[-1][1]
+        log(f"cpu leg: {cs} oracle steps in {el:.1f}s; GPU rel-L2 vs oracle {parity_curve}")
 
     # ---------------------------------------------------------------- timed leg
     dev.set_state(0, psi0)
@@ -213,6 +227,8 @@ def main():
         "step_alg_GBps": round(step_gbps, 1),
         "step_roofline_frac": round(step_gbps / PEAK_HBM_GBPS, 4),
         "parity_rel_l2_vs_oracle": parity,
+        "parity_steps": args.cpu_steps if parity is not None else 0,
+        "parity_curve": parity_curve,
         "cpu_baseline": cpu_baseline,
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
     }

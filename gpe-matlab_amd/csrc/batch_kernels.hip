@@ -18,7 +18,7 @@ template <int N, bool COLFAST, int SEQ>
 __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
   constexpr int C = 256 / Plan<N>::T;
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[N];
+  __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
   __shared__ double red[4];
   const WorkItem it = b.items[blockIdx.x];
   const TrajDev& tj = b.traj[it.traj];

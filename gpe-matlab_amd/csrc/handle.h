@@ -357,9 +357,15 @@ inline int run_block(gpe_handle* h, double time2) {
       if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
     }
   }
-  if ((rc = sweep(h, 0, gpe::S_CLOSE, base_params(h, 0, psi, V)))) return rc;
+  // closing phases fused with the x part of the block-end reductions
   FinParams f = fin_params(h, gpe::FIN_BLOCK);
-  if ((rc = launch_measure(h, 0, &f))) return rc;
+  int nb = 0;
+  if ((rc = sweep(h, 0, gpe::S_CLOSEM, base_params(h, 0, psi, V), &nb))) return rc;
+  f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
+  for (int a = 1; a < h->ndims; ++a) {
+    if ((rc = sweep(h, a, gpe::S_KE, base_params(h, a, psi, V), &nb))) return rc;
+    f.cnt[gpe::SL_KX + a] = nb;
+  }
   f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
   f.NN0 = h->NN0;
   f.tau = h->prm.decay_rate;

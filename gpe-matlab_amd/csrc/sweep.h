@@ -22,7 +22,8 @@ enum Seq : int {
   S_KE = 7,     // forward and reduce k^2/2 |.|^2
   S_ITPA = 8,   // inverse, x stored real operator N, reduce |psi|^2 (groundstate_itp.m:41-44)
   S_ITPB = 9,   // x mu, N = exp(-dt/2 (g mu^2 n + V)), store N, x N, forward (groundstate_itp.m:76,40)
-  S_NSEQ = 10
+  S_CLOSEM = 10,  // S_CLOSE, then the S_KEX reductions of the stored state in the same pass
+  S_NSEQ = 11
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,

@@ -14,12 +14,14 @@
 namespace gpe {
 
 template <int SEQ> struct SeqTraits {
-  static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_KEX || SEQ == S_ITPB;
+  static constexpr bool LOAD_V =
+      SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM || SEQ == S_KEX || SEQ == S_ITPB;
   static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE);
 };
 
 // One work-group's share of a sweep: pencils tile*C .. tile*C + C-1.  `lds`
-// holds Lds<N, C, COLFAST>::SIZE doubles, `twl` N twiddles, `red` 4 doubles;
+// holds Lds<N, C, COLFAST>::SIZE doubles, `twl` 2N entries (twiddles, then the
+// S_KIN per-index kinetic factors), `red` 4 doubles;
 // reductions go to partials[slot * maxwg + wg].
 template <int N, int C, bool COLFAST, int SEQ>
 __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, unsigned wg, double* lds,
@@ -33,6 +35,10 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   const int c = COLFAST ? tid % C : tid / T;
   const int t = COLFAST ? tid / C : tid % T;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
+  if constexpr (SEQ == S_KIN) {
+    for (int i = tid; i < N; i += 256) twl[N + i] = p.kax[i];
+    __syncthreads();  // single-pass plans have no exchange barrier before the first use
+  }
 
   const unsigned cgu = tile * (unsigned)C + (unsigned)c;
   const bool active = cgu < p.ncols;
@@ -62,7 +68,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
       v[m] = mul_cexp(x, a * p.dr * 0.5, a * p.di * 0.5, has_re);
     }
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
-  } else if constexpr (SEQ == S_MID || SEQ == S_CLOSE) {
+  } else if constexpr (SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM) {
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
     const double mu = p.st->mu;
 #pragma unroll
@@ -70,7 +76,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
       const double vm = vv[m] - mu;
       const double a = -vm - p.g * cabs2(v[m]);                  // solve_split.m:60
       double2 x = mul_cexp(v[m], a * p.dr, a * p.di, has_re);     // solve_split.m:61
-      if constexpr (SEQ == S_CLOSE) {
+      if constexpr (SEQ == S_CLOSE || SEQ == S_CLOSEM) {
         const double b = vm + p.g * cabs2(x);                      // solve_split.m:63
         x = mul_cexp(x, b * p.dr * 0.5, b * p.di * 0.5, has_re);   // solve_split.m:64
       }
@@ -87,10 +93,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   } else if constexpr (SEQ == S_INV) {
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_KIN) {
-    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    // the per-index factors share the twiddle buffer's second half (staged above)
     const double2 kc = p.kcol[cg];
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, p.kax[t + T * m]), v[m]);
+    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_KEX || SEQ == S_KE) {
     if constexpr (SEQ == S_KEX) {
@@ -138,8 +145,26 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
     }
   }
 
+  if constexpr (SEQ == S_CLOSEM) {
+    // block-end reductions of the stored real-space state (solve_split.m:65-77)
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double n = cabs2(v[m]);
+        accA += n;
+        accB += vv[m] * n;
+        accC += n * n;
+      }
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
+    }
+  }
+
   // deterministic block reductions
-  if constexpr (SEQ == S_KEX) {
+  if constexpr (SEQ == S_KEX || SEQ == S_CLOSEM) {
     const double a = block_sum(accA, red);
     const double b = block_sum(accB, red);
     const double cc = block_sum(accC, red);
@@ -170,10 +195,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
 
 template <int N, int C, bool COLFAST, int SEQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    SEQ == S_KIN ? GPE_WAVES_KIN : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE) ? GPE_WAVES_PHASE : 1)))
+    SEQ == S_KIN ? GPE_WAVES_KIN
+                 : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE || SEQ == S_CLOSEM) ? GPE_WAVES_PHASE : 1)))
     sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[N];
+  __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
   __shared__ double red[4];
   const unsigned tile = p.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
   sweep_body<N, C, COLFAST, SEQ>(p, tile, tile, lds, twl, red);

@@ -27,6 +27,7 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, const SweepParam
     switch (seq) {
       GPE_L(true, S_FWD) GPE_L(true, S_INV) GPE_L(true, S_KIN) GPE_L(true, S_OPEN) GPE_L(true, S_MID)
       GPE_L(true, S_CLOSE) GPE_L(true, S_KEX) GPE_L(true, S_KE) GPE_L(true, S_ITPA) GPE_L(true, S_ITPB)
+      GPE_L(true, S_CLOSEM)
       default: return hipErrorInvalidValue;
     }
   } else {

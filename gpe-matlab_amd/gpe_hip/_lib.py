@@ -263,13 +263,14 @@ class Device:
 
     def profile_read(self):
         """{(seq, axis): (total_ms, launches)} of the sweeps since the last read."""
-        ms = np.zeros(30)
-        cnt = np.zeros(30, dtype=np.int32)
-        self.call("gpe_profile_read", dptr(ms), cnt.ctypes.data_as(_IP), 30)
-        return {(i // 3, i % 3): (float(ms[i]), int(cnt[i])) for i in range(30) if cnt[i] > 0}
+        ms = np.zeros(33)
+        cnt = np.zeros(33, dtype=np.int32)
+        self.call("gpe_profile_read", dptr(ms), cnt.ctypes.data_as(_IP), 33)
+        return {(i // 3, i % 3): (float(ms[i]), int(cnt[i])) for i in range(33) if cnt[i] > 0}
 
 
-SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb"}
+SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb",
+             10: "closem"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
 
 

@@ -159,7 +159,7 @@ int gpe_batch_split_outer(gpe_batch* b, int j, double* time2, double* muc, doubl
 /* Measurement hooks (no reference counterpart): time every sweep launch with
  * HIP events recorded on the handle's own stream.  gpe_profile_read fills
  * ms[seq*3 + axis] (summed milliseconds) and count[seq*3 + axis] for
- * seq < 10 (the sweep sequences) and axis 0..2 (x, y, z); n >= 30. */
+ * seq < 11 (the sweep sequences) and axis 0..2 (x, y, z); n >= 33. */
 int gpe_profile_enable(gpe_handle* h, int enable);
 int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n);
 
