@@ -146,9 +146,12 @@ def main():
         to.init_state = psi0
         cpu_ck = {}
 
+        t_start = time.perf_counter()
+
         def grab(t):
             if (t.current_iter * 10) % every == 0:
                 cpu_ck[t.current_iter * 10] = t.current_state.copy()
+                log(f"  oracle at step {t.current_iter * 10} ({time.perf_counter() - t_start:.0f}s)")
 
         to.user_callback = grab
         t0 = time.perf_counter()
