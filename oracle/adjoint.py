@@ -125,3 +125,119 @@ class ControlProblem:
         for k in range(self.G.shape[0]):
             grad[k, cur_m] += np.sum(self.G[k] * gV)
         return L, grad, lam
+
+
+class SpinorRabiControl:
+    """Gradient of the final population imbalance of GPEtaskSpinor.solve_split
+    (lib/@GPEtaskSpinor/solve_split.m:1-121, gamma = 0) with respect to (Omega0, k)
+    of the coupling Omega(x) = Omega0 exp(i k X).
+
+    Loss L = (N1 - N2) / Ntotal at the end, N_j = integrate(|psi_j|^2).
+    The step is w = D(theta(u)) R u per point with R = [[c, -i s e^{ikX}],
+    [-i s e^{-ikX}, c]], c = cos(h Omega0), s = sin(h Omega0) (solve_split.m:61-65,90-91
+    with dt = i h), theta_j = h (V_j - mu + sum_k g_jk |u_k|^2) (78-83), then the
+    kinetic steps (92-94, 73-75, 97-99).  Adjoint (lambda = 2 dL/d conj(psi)):
+        mu_j = e^{i theta_j} lambda_j,   q_j = Im(conj(lambda_j) w_j),
+        a_k = h sum_j g_jk q_j,          lambda_u = R^H mu + 2 a u,
+        dL/dp += Re sum_x mu^H (dR/dp) u   for p = Omega0, k.
+    The chemical potential only multiplies both components by a common phase,
+    which the loss ignores, so it is held at the values of the forward run
+    (real part, as the device computes it).
+    """
+
+    def __init__(self, grid, V1, V2, g, ddt, nsteps, psi0, Ntotal, n_rec=10):
+        self.grid = grid
+        self.V = [V1, V2]
+        self.g = np.asarray(g, dtype=float)
+        self.h = ddt
+        self.nsteps = nsteps
+        self.n_rec = n_rec
+        self.psi0 = [psi0[0].copy(), psi0[1].copy()]
+        self.Ntotal = Ntotal
+        kk = grid.kk
+        self.ekk = np.exp(-kk * 1j * ddt)
+        self.ekkp = np.exp(-kk * 1j * ddt * 0.5)
+        self.ekkm = np.exp(kk * 1j * ddt * 0.5)
+        self.X = grid.mesh.x
+
+    def _kin(self, p, op):
+        return self.grid.ifft(op * self.grid.fft(p))
+
+    def _mu(self, phi, Om):
+        gr = self.grid
+        tot = 0.0
+        for j in range(2):
+            res = self.V[j] + self.g[j, 0] * np.abs(phi[0]) ** 2 + self.g[j, 1] * np.abs(phi[1]) ** 2
+            res = res * phi[j] + gr.lap(phi[j])
+            res = res + (Om * phi[1] if j == 0 else np.conj(Om) * phi[0])
+            tot = tot + gr.inner(phi[j], res)
+        return np.real(tot)
+
+    def forward(self, Om0, k, keep=False):
+        h = self.h
+        Om = Om0 * np.exp(1j * k * self.X)
+        c, s = np.cos(h * Om0), np.sin(h * Om0)
+        sp = 1j * s * np.exp(1j * k * self.X)
+        sm = 1j * s * np.exp(-1j * k * self.X)
+        phi = [self.psi0[0].copy(), self.psi0[1].copy()]
+        mu = self._mu(phi, Om) / self.Ntotal
+        tape = []
+        for b in range(self.nsteps // self.n_rec):
+            phi = [self._kin(p, self.ekkp) for p in phi]
+            for i in range(self.n_rec):
+                u = phi
+                n1, n2 = np.abs(u[0]) ** 2, np.abs(u[1]) ** 2
+                th1 = h * (self.V[0] - mu + self.g[0, 0] * n1 + self.g[0, 1] * n2)
+                th2 = h * (self.V[1] - mu + self.g[1, 0] * n1 + self.g[1, 1] * n2)
+                w1 = np.exp(-1j * th1) * (c * u[0] - sp * u[1])
+                w2 = np.exp(-1j * th2) * (c * u[1] - sm * u[0])
+                if keep:
+                    tape.append((u, (w1, w2), (th1, th2)))
+                phi = [self._kin(w1, self.ekk), self._kin(w2, self.ekk)]
+            phi = [self._kin(p, self.ekkm) for p in phi]
+            mu = self._mu(phi, Om) / self.Ntotal
+        return (phi, tape) if keep else phi
+
+    def loss(self, Om0, k):
+        phi = self.forward(Om0, k)
+        gr = self.grid
+        return (gr.integrate(np.abs(phi[0]) ** 2) - gr.integrate(np.abs(phi[1]) ** 2)) / self.Ntotal
+
+    def loss_and_grad(self, Om0, k):
+        gr = self.grid
+        h = self.h
+        phi, tape = self.forward(Om0, k, keep=True)
+        L = (gr.integrate(np.abs(phi[0]) ** 2) - gr.integrate(np.abs(phi[1]) ** 2)) / self.Ntotal
+        f = 2.0 * gr.weight / self.Ntotal
+        lam = [f * phi[0], -f * phi[1]]
+        c, s = np.cos(h * Om0), np.sin(h * Om0)
+        e = np.exp(1j * k * self.X)
+        sp, sm = 1j * s * e, 1j * s * np.conj(e)
+        dR0 = ((-h * s, -1j * h * c * e), (-1j * h * c * np.conj(e), -h * s))
+        dRk = ((0.0, s * self.X * e), (-s * self.X * np.conj(e), 0.0))
+        gO = gk = 0.0
+        idx = len(tape) - 1
+        for b in reversed(range(self.nsteps // self.n_rec)):
+            lam = [self._kin(l, np.conj(self.ekkm)) for l in lam]
+            for i in reversed(range(self.n_rec)):
+                u, w, th = tape[idx]
+                idx -= 1
+                lam = [self._kin(l, np.conj(self.ekk)) for l in lam]
+                m1, m2 = np.exp(1j * th[0]) * lam[0], np.exp(1j * th[1]) * lam[1]
+                q1 = np.imag(np.conj(lam[0]) * w[0])
+                q2 = np.imag(np.conj(lam[1]) * w[1])
+                a1 = h * (self.g[0, 0] * q1 + self.g[1, 0] * q2)
+                a2 = h * (self.g[0, 1] * q1 + self.g[1, 1] * q2)
+                for dR, which in ((dR0, 0), (dRk, 1)):
+                    t1 = dR[0][0] * u[0] + dR[0][1] * u[1]
+                    t2 = dR[1][0] * u[0] + dR[1][1] * u[1]
+                    val = np.sum(np.real(np.conj(m1) * t1 + np.conj(m2) * t2))
+                    if which == 0:
+                        gO += val
+                    else:
+                        gk += val
+                l1 = np.conj(c) * m1 + np.conj(-sm) * m2 + 2 * a1 * u[0]
+                l2 = np.conj(-sp) * m1 + np.conj(c) * m2 + 2 * a2 * u[1]
+                lam = [l1, l2]
+            lam = [self._kin(l, np.conj(self.ekkp)) for l in lam]
+        return L, np.array([gO, gk])

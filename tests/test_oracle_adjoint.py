@@ -54,3 +54,37 @@ def test_adjoint_directional_derivative():
     eps = 1e-6
     fd = (cp.loss(c + eps * v, psi_t) - cp.loss(c - eps * v, psi_t)) / (2 * eps)
     assert abs(fd - np.sum(grad * v)) <= 1e-6 * abs(fd)
+
+
+def spinor_problem(n=16, nsteps=30, seed=3):
+    from oracle.adjoint import SpinorRabiControl
+    grid = oracle.grid2d(5, n, 5, n)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2) * np.exp(1j * rng.uniform(0, 0.3, grid.size))
+    b = 0.5 * np.exp(-((grid.mesh.x - 0.5) ** 2 + grid.mesh.y ** 2) / 2) + 0j
+    N = grid.integrate(np.abs(a) ** 2 + np.abs(b) ** 2)
+    g = [[0.5, 0.45], [0.45, 0.6]]
+    return SpinorRabiControl(grid, V, V + 0.2 * grid.mesh.y, g, 0.01, nsteps, [a, b], N), grid, g, [a, b], N
+
+
+def test_spinor_control_forward_is_solve_split():
+    cp, grid, g, psi0, N = spinor_problem()
+    t = oracle.GPEtaskSpinor(grid, cp.V)
+    t.g = np.array(g)
+    t.coupling = 0.7 * np.exp(1j * 1.3 * grid.mesh.x)
+    t.Ntotal = N
+    t.init_state = psi0
+    ref = t.solve_split(0.01, 30, 1)
+    out = cp.forward(0.7, 1.3)
+    assert max(rel_l2(out[0], ref[0]), rel_l2(out[1], ref[1])) < 1e-12
+
+
+def test_spinor_adjoint_matches_central_fd():
+    cp, *_ = spinor_problem()
+    L, grad = cp.loss_and_grad(0.7, 1.3)
+    h = 1e-6
+    fdO = (cp.loss(0.7 + h, 1.3) - cp.loss(0.7 - h, 1.3)) / (2 * h)
+    fdk = (cp.loss(0.7, 1.3 + h) - cp.loss(0.7, 1.3 - h)) / (2 * h)
+    assert abs(grad[0] - fdO) <= 1e-6 * abs(fdO) + 1e-9, (grad, fdO)
+    assert abs(grad[1] - fdk) <= 1e-6 * abs(fdk) + 1e-9, (grad, fdk)
