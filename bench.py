@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
@@ -454,11 +455,44 @@ def bench_c5(args):
     rate = steps / el
     gbps = 288 * n ** 3 * rate / 1e9
     log(f"c5: {steps} spinor steps in {el:.3f}s -> {rate:.1f} steps/s, {gbps:.0f} GB/s algorithmic")
+    grad_info = {}
+    if not args.no_adjoint:
+        grad_info = bench_c5_adjoint(t, log)
     _line("spinor split-steps/s (GPEtaskSpinor 3D 192^3, 2 components)", round(rate, 2), "split-steps/s", steps, 10, el,
           True, {"workload": "c5: grid3d(12,192,...) V1=V2=0.5(X^2+Y^2+Z^2), g=[0.02 0.018;0.018 0.02], "
                              "Omega=0.5 exp(2iX), Ntotal=1e6, TF split 50/50, dt=0.002, n_recalc=10", "grid": [n] * 3},
           step_alg_GBps=round(gbps, 1), step_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
-          parity_rel_l2_vs_oracle_10_steps=parity)
+          parity_rel_l2_vs_oracle_10_steps=parity, **grad_info)
+
+
+def bench_c5_adjoint(t, log, omega0=0.5, k=2.0, nsteps=1000, ddt=0.002, h=1e-6):
+    """Gradient of (N1 - N2)/Ntotal after 1000 steps w.r.t. (Omega0, k) of
+    Omega(x) = Omega0 exp(i k X): cost against the plain forward, and central FD."""
+    from gpe_hip.adjoint import RabiControl
+    ctl = RabiControl(t, ddt, nsteps)
+    dev = ctl._prepare()
+    dev.rabi_forward(ddt, nsteps, omega0, k, want_state=False)   # warm-up (buffers, tables)
+    dev.rabi_grad(ddt, nsteps, omega0, k)
+    t0 = time.perf_counter()
+    L, _ = dev.rabi_forward(ddt, nsteps, omega0, k, want_state=False)
+    ef = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Lg, g = dev.rabi_grad(ddt, nsteps, omega0, k)
+    eg = time.perf_counter() - t0
+    log(f"c5 adjoint: forward {ef:.3f}s, gradient {eg:.3f}s (ratio {eg / ef:.2f}); L={L:.12f} grad={g}")
+    fd = []
+    for i in range(2):
+        p = [omega0, k]
+        p[i] += h
+        Lp = dev.rabi_forward(ddt, nsteps, *p, want_state=False)[0]
+        p[i] -= 2 * h
+        Lm = dev.rabi_forward(ddt, nsteps, *p, want_state=False)[0]
+        fd.append((Lp - Lm) / (2 * h))
+    rel = [abs(g[i] - fd[i]) / abs(fd[i]) for i in range(2)]
+    log(f"c5 adjoint vs central FD (h={h}): fd={fd}, rel={rel}")
+    return dict(adjoint_loss=L, adjoint_grad=[float(x) for x in g], adjoint_fd=fd, adjoint_fd_rel=rel,
+                adjoint_forward_s=round(ef, 4), adjoint_grad_s=round(eg, 4),
+                adjoint_over_forward=round(eg / ef, 3), adjoint_same_loss=bool(L == Lg))
 
 
 if __name__ == "__main__":

@@ -63,6 +63,7 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     dd[a] = x[1] - x[0];
     h->ax[a].k = kvec(x, dd[a]);
     h->ax[a].n = nn[a];
+    h->ax[a].x = x;
   }
   h->dx = dd[0];
   h->dy = dd[1];
@@ -149,6 +150,17 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->adj_grad);
   (void)hipFree(h->kcol_c);
   (void)hipFree(h->kax_c);
+  for (int i = 0; i < 2; ++i) (void)hipFree(h->rb_lam[i]);
+  (void)hipFree(h->rb_ckpt);
+  (void)hipFree(h->rb_tape);
+  (void)hipFree(h->rb_mu);
+  (void)hipFree(h->rb_xc);
+  (void)hipFree(h->rb_gacc);
+  (void)hipFree(h->rb_out);
+  for (int i = 0; i < 3; ++i) {
+    (void)hipFree(h->kcol_rc[i]);
+    (void)hipFree(h->kax_rc[i]);
+  }
   (void)hipFree(h->partials);
   (void)hipFree(h->st);
   (void)hipFree(h->MUdev);

@@ -28,6 +28,7 @@ struct Axis {
   double2* tw = nullptr;      // forward twiddles
   double* keax = nullptr;     // k^2/2/n
   std::vector<double> k;      // wave vectors (grid*/init.m k layout)
+  std::vector<double> x;      // coordinates
 };
 
 }  // namespace gpe_detail
@@ -79,6 +80,19 @@ struct gpe_handle {
   int adj_gradcap = 0;
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
+  // spinor Rabi adjoint (gpe_rabi_forward, gpe_rabi_grad)
+  double2* rb_lam[2] = {nullptr, nullptr};
+  double2* rb_ckpt = nullptr;     // checkpoints, 2 components each
+  long long rb_nckpt = 0;
+  double2* rb_tape = nullptr;     // pre-step states of one segment, 2 components per step
+  long long rb_ntape = 0;
+  double* rb_mu = nullptr;        // mu at each block start
+  long long rb_nmu = 0;
+  double* rb_xc = nullptr;        // x coordinates
+  double* rb_gacc = nullptr;      // 2 x maxwg gradient accumulators
+  double* rb_out = nullptr;       // 2 reduced gradient components
+  double2* kcol_rc[3] = {nullptr, nullptr, nullptr};   // conj(exp(-kk dt/2)), conj(exp(-kk dt)),
+  double2* kax_rc[3] = {nullptr, nullptr, nullptr};    // conj(exp(-kk dt) exp(kk dt/2))
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)

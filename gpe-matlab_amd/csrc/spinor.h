@@ -1,6 +1,20 @@
 // Two-component (GPEtaskSpinor) drivers.
 #pragma once
+#include "sweep.h"
 struct gpe_handle;
+
+namespace gpe_spinor {
+gpe::SpinorParams spinor_params(gpe_handle* h);
+int spinor_sweep(gpe_handle* h, int seq, const gpe::SpinorParams& p, int* nb);
+// Coefficients of a scalar coupling (solve_split.m:61-65).
+void rabi_scalars(gpe_handle* h);
+// y/z sweeps of one step on one array with the kinetic table pair (kcol, kax).
+int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax);
+// One block of n_rec steps.  tape (if set) receives the real-space state of both
+// components before each step (2 * n_rec arrays); reduce = false skips the
+// block-end chemical potential and renormalisation.
+int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce);
+}  // namespace gpe_spinor
 
 namespace gpe {
 int spinor_measure(gpe_handle* h, double out[5]);

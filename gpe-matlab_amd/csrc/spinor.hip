@@ -14,7 +14,7 @@
 
 using namespace gpe_detail;
 
-namespace {
+namespace gpe_spinor {
 
 gpe::SpinorParams spinor_params(gpe_handle* h) {
   gpe::SpinorParams p{};
@@ -49,18 +49,19 @@ gpe::SpinorParams spinor_params(gpe_handle* h) {
   return p;
 }
 
-int spinor_sweep(gpe_handle* h, int seq, int* nb = nullptr) {
+int spinor_sweep(gpe_handle* h, int seq, const gpe::SpinorParams& p, int* nb) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof) {
     e0 = prof_event(h);
     e1 = prof_event(h);
     if (e0) hipEventRecord(e0, h->stream);
   }
-  hipError_t e = gpe::launch_spinor_sweep(h->ax[0].n, seq, spinor_params(h), h->stream, nb);
+  hipError_t e = gpe::launch_spinor_sweep(h->ax[0].n, seq, p, h->stream, nb);
   if (h->prof && e0 && e1) {
     hipEventRecord(e1, h->stream);
-    // spinor sweeps are reported in the (seq, axis) slots of S_MID / S_KEX on x
-    h->prof_recs.push_back({(seq == gpe::SP_PR ? gpe::S_MID : gpe::S_KEX) * 3 + 0, {e0, e1}});
+    // spinor sweeps are reported in the (seq, axis) slots of S_MID / S_KEX / S_CLOSE on x
+    const int slot = seq == gpe::SP_PR ? gpe::S_MID : seq == gpe::SP_KEX ? gpe::S_KEX : gpe::S_CLOSE;
+    h->prof_recs.push_back({slot * 3 + 0, {e0, e1}});
   }
   if (e != hipSuccess) return fail(h, GPE_EHIP, "spinor sweep launch: %s", hipGetErrorString(e));
   return GPE_OK;
@@ -84,7 +85,7 @@ void rabi_scalars(gpe_handle* h) {
 // Reductions for ncur and mu on the real-space state of both components.
 int spinor_reduce(gpe_handle* h, FinParams* f) {
   int rc, nb = 0;
-  if ((rc = spinor_sweep(h, gpe::SP_KEX, &nb))) return rc;
+  if ((rc = spinor_sweep(h, gpe::SP_KEX, spinor_params(h), &nb))) return rc;
   for (int s : {gpe::SL_A, gpe::SL_B, gpe::SL_C, gpe::SL_C12, gpe::SL_C2, gpe::SL_X, gpe::SL_KX, gpe::SL_KX2})
     f->cnt[s] = nb;
   for (int comp = 0; comp < 2; ++comp)
@@ -101,11 +102,10 @@ int spinor_reduce(gpe_handle* h, FinParams* f) {
   return GPE_OK;
 }
 
-// y forward, z kinetic, y inverse (3D) or y kinetic (2D) for one component
+// y forward, z kinetic, y inverse (3D) or y kinetic (2D) for one array
 // with the kinetic table pair (kcol, kax).
-int spinor_rest_of_step(gpe_handle* h, int comp, double2* kcol, double2* kax) {
+int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax) {
   int rc;
-  double2* psi = h->psi[comp];
   if (h->ndims == 3) {
     if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, nullptr)))) return rc;
     SweepParams k = base_params(h, 2, psi, nullptr);
@@ -122,24 +122,30 @@ int spinor_rest_of_step(gpe_handle* h, int comp, double2* kcol, double2* kax) {
   return GPE_OK;
 }
 
-int spinor_block(gpe_handle* h, double time2) {
+int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
   int rc;
   // open: x forward (with the pending renormalisation), then exp(-kk dt/2)
   for (int comp = 0; comp < 2; ++comp) {
     SweepParams p = base_params(h, 0, h->psi[comp], nullptr);
     p.apply_scale = 1;
     if ((rc = sweep(h, 0, gpe::S_FWD, p))) return rc;
-    if ((rc = spinor_rest_of_step(h, comp, h->kcol_sp[0], h->kax_sp[0]))) return rc;
+    if ((rc = spinor_rest_of_step(h, h->psi[comp], h->kcol_sp[0], h->kax_sp[0]))) return rc;
   }
   for (int i = 0; i < h->n_rec; ++i) {
-    if ((rc = spinor_sweep(h, gpe::SP_PR))) return rc;
+    gpe::SpinorParams p = spinor_params(h);
+    if (tape) {
+      p.ust1 = tape + (size_t)(2 * i) * h->npts;
+      p.ust2 = tape + (size_t)(2 * i + 1) * h->npts;
+    }
+    if ((rc = spinor_sweep(h, gpe::SP_PR, p, nullptr))) return rc;
     const bool last = i + 1 == h->n_rec;
     for (int comp = 0; comp < 2; ++comp)
-      if ((rc = spinor_rest_of_step(h, comp, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax)))
+      if ((rc = spinor_rest_of_step(h, h->psi[comp], last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax)))
         return rc;
   }
   for (int comp = 0; comp < 2; ++comp)
     if ((rc = sweep(h, 0, gpe::S_INV, base_params(h, 0, h->psi[comp], nullptr)))) return rc;
+  if (!reduce) return GPE_OK;
   FinParams f = fin_params(h, gpe::FIN_SPINOR);
   if ((rc = spinor_reduce(h, &f))) return rc;
   f.renorm = h->prm.gamma > 0 ? 1 : 0;
@@ -150,7 +156,9 @@ int spinor_block(gpe_handle* h, double time2) {
   return GPE_OK;
 }
 
-}  // namespace
+}  // namespace gpe_spinor
+
+using namespace gpe_spinor;
 
 namespace gpe {
 
@@ -212,7 +220,7 @@ int spinor_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_o
 int spinor_split_block(gpe_handle* h, double time2, double* muc, double* ncur) {
   int rc;
   rabi_scalars(h);
-  if ((rc = spinor_block(h, time2))) return rc;
+  if ((rc = spinor_block(h, time2, nullptr, true))) return rc;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
   if (muc) *muc = s.muc;
@@ -229,7 +237,7 @@ int spinor_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, dou
   const int nblocks = h->niter_inner / h->n_rec;              // 67 (colon truncates)
   for (int jj = 1; jj <= nblocks; ++jj) {
     time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 69
-    if ((rc = spinor_block(h, time2))) return rc;
+    if ((rc = spinor_block(h, time2, nullptr, true))) return rc;
   }
   DevState s;
   if ((rc = read_state(h, &s))) return rc;

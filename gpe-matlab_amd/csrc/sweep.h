@@ -69,7 +69,7 @@ struct SweepParams {
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).
-enum SpSeq : int { SP_PR = 0, SP_KEX = 1 };
+enum SpSeq : int { SP_PR = 0, SP_KEX = 1, SP_ADJ = 2 };
 
 struct SpinorParams {
   double2* psi1;
@@ -88,6 +88,12 @@ struct SpinorParams {
   double2 cosom, sinomm, sinomp;  // scalar Rabi coefficients (solve_split.m:61-65), cpl == null
   double nlincpl;            // nonlinear coupling (solve_split.m:84-89)
   const double* keax;        // k^2/2/n along x (SP_KEX)
+  // Rabi adjoint (gamma = 0, Omega(x) = om0 exp(i kw x)):
+  double2* ust1;             // SP_PR: if set, store the real-space state before the step there;
+  double2* ust2;             // SP_ADJ: read that stored state back
+  const double* xc;          // SP_ADJ: x coordinate per x index
+  double om0, kw;            // SP_ADJ: coupling amplitude and wave number
+  double* gacc;              // SP_ADJ: per-work-group accumulators of dL/d om0 [0, maxwg) and dL/d kw
 };
 
 // Launch one sweep; returns hipErrorInvalidValue for an unsupported length.

@@ -33,6 +33,11 @@ class ControlArgs(C.Structure):
                 ("checkpoint_blocks", C.c_int)]
 
 
+class RabiArgs(C.Structure):
+    _fields_ = [("ddt", C.c_double), ("nsteps", C.c_int), ("checkpoint_blocks", C.c_int), ("omega0", C.c_double),
+                ("k", C.c_double)]
+
+
 class ItpArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("min_iter", C.c_int),
                 ("adjust_stepsize", C.c_int), ("nargout", C.c_int)]
@@ -63,6 +68,8 @@ SIGNATURES = {
     "gpe_control_set": [_P, C.c_int, _DP, _DP],
     "gpe_control_forward": [_P, C.POINTER(ControlArgs), _DP, _DP],
     "gpe_adjoint_grad": [_P, C.POINTER(ControlArgs), _DP, _DP, _DP, _DP, _DP],
+    "gpe_rabi_forward": [_P, C.POINTER(RabiArgs), _DP, _DP],
+    "gpe_rabi_grad": [_P, C.POINTER(RabiArgs), _DP, _DP],
     "gpe_batch_create": [C.c_int, C.POINTER(GridDesc), C.POINTER(C.c_void_p)],
     "gpe_batch_destroy": [_P],
     "gpe_batch_set": [_P, C.c_int, C.POINTER(Params), _DP, _DP],
@@ -257,6 +264,24 @@ class Device:
         self.call("gpe_adjoint_grad", C.byref(a), dptr(c), t.view(np.float64).ctypes.data_as(_DP), C.byref(loss),
                   dptr(grad), C.byref(rerr))
         return loss.value, grad.reshape(np.asarray(coeffs).shape, order="F"), rerr.value
+
+    def rabi_forward(self, ddt, nsteps, omega0, k, ckpt=0, want_state=True):
+        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k))
+        loss = C.c_double()
+        if not want_state:
+            self.call("gpe_rabi_forward", C.byref(a), C.byref(loss), None)
+            return loss.value, None
+        out = np.empty(2 * self.npts, dtype=np.complex128)
+        self.call("gpe_rabi_forward", C.byref(a), C.byref(loss), out.view(np.float64).ctypes.data_as(_DP))
+        return loss.value, [from_colmajor_c128(out[:self.npts], self.shape),
+                            from_colmajor_c128(out[self.npts:], self.shape)]
+
+    def rabi_grad(self, ddt, nsteps, omega0, k, ckpt=0):
+        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k))
+        loss = C.c_double()
+        grad = np.zeros(2)
+        self.call("gpe_rabi_grad", C.byref(a), C.byref(loss), dptr(grad))
+        return loss.value, grad
 
     def profile_enable(self, on=True):
         self.call("gpe_profile_enable", int(bool(on)))

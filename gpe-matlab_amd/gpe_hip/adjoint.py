@@ -74,3 +74,69 @@ class SplitStepControl:
                 return gL * g
 
         return _Loss
+
+
+class RabiControl:
+    """Gradient of the final population imbalance L = (N1 - N2) / Ntotal of a
+    GPEtaskSpinor.solve_split run (gamma = 0) with respect to (Omega0, k) of the
+    coupling Omega(x) = Omega0 exp(i k x) (gpe_rabi_grad in include/gpe_hip.h).
+
+    The run starts from ``task.init_state`` with ``task.g``, ``task.Vtrap`` (static
+    potentials) and ``task.n_recalc``; ``nsteps`` must be a multiple of n_recalc.
+    The two-component step is not explicitly invertible, so the backward pass
+    recomputes checkpoint segments of about sqrt(nsteps) steps from stored states.
+    """
+
+    def __init__(self, task, ddt, nsteps, checkpoint_blocks=0):
+        if task.gamma != 0:
+            raise ValueError("the Rabi adjoint needs gamma = 0")
+        if not task.Ntotal > 0:
+            raise ValueError("the Rabi adjoint needs Ntotal > 0")
+        if task.Vtd is not None:
+            raise ValueError("the Rabi adjoint supports static potentials only")
+        self.task = task
+        self.ddt = float(ddt)
+        self.nsteps = int(nsteps)
+        self.ckpt = int(checkpoint_blocks)
+
+    def _prepare(self):
+        t = self.task
+        dev = t._device()
+        t._set_params(t.Ntotal)
+        t._upload_potential(t.getVtotal(0))
+        psi0 = t.init_state
+        t._upload(psi0)
+        t._dev_state_valid = False
+        return dev
+
+    def forward(self, omega0, k):
+        """(L, [psi1(T), psi2(T)])"""
+        return self._prepare().rabi_forward(self.ddt, self.nsteps, omega0, k, self.ckpt)
+
+    def loss(self, omega0, k):
+        return self._prepare().rabi_forward(self.ddt, self.nsteps, omega0, k, self.ckpt, want_state=False)[0]
+
+    def loss_and_grad(self, omega0, k):
+        """(L, array([dL/dOmega0, dL/dk]))"""
+        return self._prepare().rabi_grad(self.ddt, self.nsteps, omega0, k, self.ckpt)
+
+    def autograd_function(self):
+        """A torch.autograd.Function mapping a float64 tensor (Omega0, k) to L."""
+        import torch
+
+        ctl = self
+
+        class _Loss(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, p):
+                v = p.detach().cpu().numpy()
+                L, g = ctl.loss_and_grad(float(v[0]), float(v[1]))
+                ctx.save_for_backward(torch.as_tensor(g, dtype=p.dtype, device=p.device))
+                return torch.as_tensor(L, dtype=p.dtype, device=p.device)
+
+            @staticmethod
+            def backward(ctx, gL):
+                (g,) = ctx.saved_tensors
+                return gL * g
+
+        return _Loss

@@ -137,6 +137,28 @@ int gpe_control_forward(gpe_handle* h, const gpe_control_args* a, const double* 
 int gpe_adjoint_grad(gpe_handle* h, const gpe_control_args* a, const double* coeffs, const double* psi_target,
                      double* loss, double* grad, double* recon_err);
 
+/* ------------------------------------------------------------------ Rabi adjoint
+ * (no reference counterpart: north-star form 3 on the two-component step).
+ * Forward model: GPEtaskSpinor.solve_split (lib/@GPEtaskSpinor/solve_split.m) with
+ * gamma = 0, nlincpl = 0, Ntotal > 0, nsteps steps of ddt in blocks of n_recalc,
+ * and the coupling replaced by Omega(x) = omega0 exp(i k x).  Loss: the final
+ * population imbalance (N1 - N2) / Ntotal.  Both calls start from the uploaded
+ * two-component state and leave it in place; they leave the handle's coupling set
+ * to that Omega(x), and a later solve_split starts with gpe_split_begin.
+ * The gradient recomputes checkpoint segments (about sqrt(T) steps each) and
+ * needs about (nsteps / segments + 2 * segments + 2) state-sized arrays of HBM. */
+typedef struct {
+  double ddt;
+  int nsteps;             /* a multiple of n_recalc                         */
+  int checkpoint_blocks;  /* blocks per segment, 0: about sqrt(nsteps) steps */
+  double omega0;
+  double k;
+} gpe_rabi_args;
+/* psi_out (may be NULL): 2 * npts complex values, component 1 then component 2 */
+int gpe_rabi_forward(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* psi_out);
+/* grad[0] = dL/d omega0, grad[1] = dL/d k */
+int gpe_rabi_grad(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* grad);
+
 /* ------------------------------------------------------------------ ensemble
  * (no reference counterpart: north-star form 2).  A batch advances many
  * independent 2D GPEtask.solve_split runs together; each trajectory has its own
