@@ -490,7 +490,20 @@ def bench_c5_adjoint(t, log, omega0=0.5, k=2.0, nsteps=1000, ddt=0.002, h=1e-6):
         fd.append((Lp - Lm) / (2 * h))
     rel = [abs(g[i] - fd[i]) / abs(fd[i]) for i in range(2)]
     log(f"c5 adjoint vs central FD (h={h}): fd={fd}, rel={rel}")
+    # fourth-order central stencil at H = 1e-3: truncation ~H^4, rounding ~eps_L / H
+    H = 1e-3
+    fd4 = []
+    for i in range(2):
+        vals = []
+        for m in (2, 1, -1, -2):
+            p = [omega0, k]
+            p[i] += m * H
+            vals.append(dev.rabi_forward(ddt, nsteps, *p, want_state=False)[0])
+        fd4.append((-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * H))
+    rel4 = [abs(g[i] - fd4[i]) / abs(fd4[i]) for i in range(2)]
+    log(f"c5 adjoint vs 4th-order FD (H={H}): fd={fd4}, rel={rel4}")
     return dict(adjoint_loss=L, adjoint_grad=[float(x) for x in g], adjoint_fd=fd, adjoint_fd_rel=rel,
+                adjoint_fd4=fd4, adjoint_fd4_H=H, adjoint_fd4_rel=rel4,
                 adjoint_forward_s=round(ef, 4), adjoint_grad_s=round(eg, 4),
                 adjoint_over_forward=round(eg / ef, 3), adjoint_same_loss=bool(L == Lg))
 

@@ -5,13 +5,15 @@
 //
 // The spinor step is not explicitly invertible (the Rabi rotation mixes the
 // components before the density-dependent phase), so the backward pass does not
-// reconstruct the state: the forward pass stores a checkpoint of both components
-// every `checkpoint_blocks` blocks (about sqrt(T) steps) and the chemical
-// potential of every block; the backward pass recomputes one segment at a time,
-// storing the real-space state before each step, then runs the transposed steps
-// of that segment in reverse.  The chemical potential multiplies both components
-// by a common phase and does not change L, so it is replayed from the forward
-// run and not differentiated.
+// reconstruct the state.  The run is cut into segments of `checkpoint_blocks`
+// blocks (about sqrt(T) steps).  The forward pass stores the real-space state
+// before each step ("tape") for as many trailing segments as the tape budget
+// holds, a checkpoint of both components at the start of every other segment, and
+// the chemical potential of every block.  The backward pass walks the segments in
+// reverse, recomputing the tape of a segment without one from its checkpoint, and
+// runs the transposed steps of each segment in reverse.  The chemical potential
+// multiplies both components by a common phase and does not change L, so it is
+// replayed from the forward run and not differentiated.
 #include <algorithm>
 #include <cmath>
 
@@ -35,8 +37,10 @@ __global__ void __launch_bounds__(256) rabi_coupling_kernel(double2* __restrict_
   }
 }
 
-// Partials of sum |psi1|^2 (slot A) and sum |psi2|^2 (slot B); with lam1 set, also
-// the final adjoint lam1 = f psi1, lam2 = -f psi2 (2 dL/d conj(psi)).
+// Partials of sum (|psi1|^2 - |psi2|^2) (slot A, differenced per point, so a small
+// imbalance of two large populations keeps its digits) and sum (|psi1|^2 + |psi2|^2)
+// (slot B); with lam1 set, also the final adjoint lam1 = f psi1, lam2 = -f psi2
+// (2 dL/d conj(psi)).
 __global__ void __launch_bounds__(256) rabi_final_kernel(const double2* __restrict__ p1, const double2* __restrict__ p2,
                                                          double2* __restrict__ lam1, double2* __restrict__ lam2,
                                                          double f, long long n, double* partials, int maxwg) {
@@ -44,8 +48,9 @@ __global__ void __launch_bounds__(256) rabi_final_kernel(const double2* __restri
   double s1 = 0.0, s2 = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double2 a = p1[i], b = p2[i];
-    s1 += a.x * a.x + a.y * a.y;
-    s2 += b.x * b.x + b.y * b.y;
+    const double n1 = a.x * a.x + a.y * a.y, n2 = b.x * b.x + b.y * b.y;
+    s1 += n1 - n2;
+    s2 += n1 + n2;
     if (lam1) {
       lam1[i] = make_double2(f * a.x, f * a.y);
       lam2[i] = make_double2(-f * b.x, -f * b.y);
@@ -78,6 +83,7 @@ struct Plan {
   int B = 0;     // blocks of n_recalc steps
   int cpb = 1;   // blocks per checkpoint segment
   int nck = 0;   // segments
+  int keep = 1;  // trailing segments whose pre-step states the forward pass stores
 };
 
 int check_args(gpe_handle* h, const gpe_rabi_args* a, Plan* c) {
@@ -111,7 +117,23 @@ int grow(gpe_handle* h, T** p, long long* cap, long long need) {
   return GPE_OK;
 }
 
-int ensure_buffers(gpe_handle* h, const Plan& c, bool grad) {
+// Segments whose tape fits the budget: tape_gib > 0 GiB, 0 = half the free device
+// memory, < 0 = one segment (the minimum the recomputation needs).
+int plan_tape(gpe_handle* h, const gpe_rabi_args* a, Plan* c) {
+  const double seg = (double)c->cpb * h->prm.n_recalc * 2.0 * (double)h->npts * sizeof(double2);
+  double budget = 0.0;
+  if (a->tape_gib > 0) {
+    budget = a->tape_gib * 1073741824.0;
+  } else if (a->tape_gib == 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(h, hipMemGetInfo(&fr, &tot));
+    budget = 0.5 * ((double)fr + (double)h->rb_ntape * sizeof(double2));
+  }
+  c->keep = (int)std::min<double>(c->nck, std::max(1.0, std::floor(budget / seg)));
+  return GPE_OK;
+}
+
+int ensure_buffers(gpe_handle* h, Plan& c, const gpe_rabi_args* a, bool grad) {
   int rc;
   const long long n = h->npts;
   if ((rc = grow(h, &h->rb_ckpt, &h->rb_nckpt, (grad ? c.nck : 1) * 2 * n))) return rc;
@@ -122,7 +144,13 @@ int ensure_buffers(gpe_handle* h, const Plan& c, bool grad) {
     HIPCHK(h, hipMemcpy(h->rb_xc, h->ax[0].x.data(), h->nx * sizeof(double), hipMemcpyHostToDevice));
   }
   if (!grad) return GPE_OK;
-  if ((rc = grow(h, &h->rb_tape, &h->rb_ntape, (long long)c.cpb * h->prm.n_recalc * 2 * n))) return rc;
+  if ((rc = plan_tape(h, a, &c))) return rc;
+  const long long seg = (long long)c.cpb * h->prm.n_recalc * 2 * n;
+  if ((rc = grow(h, &h->rb_tape, &h->rb_ntape, c.keep * seg))) {
+    if (rc != GPE_ENOMEM || c.keep == 1) return rc;
+    c.keep = 1;   // the budget did not fit after all: store one segment, recompute the rest
+    if ((rc = grow(h, &h->rb_tape, &h->rb_ntape, seg))) return rc;
+  }
   for (int i = 0; i < 2; ++i)
     if (!h->rb_lam[i]) HIPCHK(h, hipMalloc(&h->rb_lam[i], n * sizeof(double2)));
   if (!h->rb_gacc) HIPCHK(h, hipMalloc(&h->rb_gacc, 2 * (size_t)h->maxwg * sizeof(double)));
@@ -164,19 +192,25 @@ int begin(gpe_handle* h, const gpe_rabi_args* a) {
   return GPE_OK;
 }
 
-double2* tape_of(gpe_handle* h, int block_in_segment) {
-  return h->rb_tape + (size_t)block_in_segment * h->prm.n_recalc * 2 * h->npts;
+// Tape of block b: the last c.keep segments have their own slots; the segments
+// before them are recomputed into slot 0 once those are consumed.
+double2* tape_of(gpe_handle* h, const Plan& c, int b) {
+  const int seg = b / c.cpb;
+  const int slot = seg >= c.nck - c.keep ? seg - (c.nck - c.keep) : 0;
+  return h->rb_tape + ((size_t)slot * c.cpb + (size_t)(b % c.cpb)) * h->prm.n_recalc * 2 * h->npts;
 }
 
-// Forward run; with grad, keeps the segment checkpoints and the last segment's tape.
+// Forward run; with grad, keeps the checkpoints of the segments it does not store
+// and the tape of the last c.keep segments.
 int forward(gpe_handle* h, const Plan& c, bool grad) {
   int rc;
   for (int b = 0; b < c.B; ++b) {
-    if (grad && b % c.cpb == 0 && b > 0)
-      if ((rc = save_state(h, h->rb_ckpt + (size_t)(b / c.cpb) * 2 * h->npts))) return rc;
+    const int seg = b / c.cpb;
+    const bool stored = grad && seg >= c.nck - c.keep;
+    if (grad && !stored && b % c.cpb == 0 && b > 0)
+      if ((rc = save_state(h, h->rb_ckpt + (size_t)seg * 2 * h->npts))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->rb_mu + b, &h->st->mu, sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    double2* tape = grad && b / c.cpb == c.nck - 1 ? tape_of(h, b % c.cpb) : nullptr;
-    if ((rc = spinor_block(h, 0.0, tape, true))) return rc;
+    if ((rc = spinor_block(h, 0.0, stored ? tape_of(h, c, b) : nullptr, true))) return rc;
   }
   return GPE_OK;
 }
@@ -195,7 +229,7 @@ int final_loss(gpe_handle* h, bool lam, double* loss) {
   HIPCHK(h, gpe::launch_finalize(fp, h->stream));
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
-  if (loss) *loss = (s.sA * h->w - s.sB * h->w) / h->prm.Ntotal;
+  if (loss) *loss = s.sA * h->w / h->prm.Ntotal;
   return GPE_OK;
 }
 
@@ -250,7 +284,7 @@ int gpe_rabi_forward(gpe_handle* h, const gpe_rabi_args* a, double* loss, double
   if (!h) return GPE_EINVAL;
   if ((rc = ensure_ready(h))) return rc;
   if ((rc = check_args(h, a, &c))) return rc;
-  if ((rc = ensure_buffers(h, c, false))) return rc;
+  if ((rc = ensure_buffers(h, c, a, false))) return rc;
   if ((rc = materialize(h))) return rc;
   if ((rc = save_state(h, h->rb_ckpt))) return rc;
   if ((rc = begin(h, a))) return rc;
@@ -269,7 +303,7 @@ int gpe_rabi_grad(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* g
   if (!h || !grad) return h ? fail(h, GPE_EINVAL, "null grad") : GPE_EINVAL;
   if ((rc = ensure_ready(h))) return rc;
   if ((rc = check_args(h, a, &c))) return rc;
-  if ((rc = ensure_buffers(h, c, true))) return rc;
+  if ((rc = ensure_buffers(h, c, a, true))) return rc;
   if ((rc = materialize(h))) return rc;
   if ((rc = save_state(h, h->rb_ckpt))) return rc;
   if ((rc = begin(h, a))) return rc;
@@ -283,16 +317,16 @@ int gpe_rabi_grad(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* g
   int nb = 0;
   for (int j = c.nck - 1; j >= 0; --j) {
     const int b0 = j * c.cpb, b1 = std::min(c.B, b0 + c.cpb);
-    if (j != c.nck - 1) {   // recompute the segment from its checkpoint, keeping the pre-step states
+    if (j < c.nck - c.keep) {   // recompute the segment from its checkpoint, keeping the pre-step states
       if ((rc = load_state(h, h->rb_ckpt + (size_t)j * 2 * h->npts))) return rc;
       for (int b = b0; b < b1; ++b) {
         if ((rc = set_mu(h, b))) return rc;
-        if ((rc = spinor_block(h, 0.0, tape_of(h, b - b0), false))) return rc;
+        if ((rc = spinor_block(h, 0.0, tape_of(h, c, b), false))) return rc;
       }
     }
     for (int b = b1 - 1; b >= b0; --b) {
       if ((rc = set_mu(h, b))) return rc;
-      if ((rc = adjoint_block(h, a, tape_of(h, b - b0), &nb))) return rc;
+      if ((rc = adjoint_block(h, a, tape_of(h, c, b), &nb))) return rc;
     }
   }
   hipLaunchKernelGGL(gpe::rabi_grad_finalize_kernel, dim3(1), dim3(256), 0, h->stream, h->rb_gacc, h->maxwg, nb,

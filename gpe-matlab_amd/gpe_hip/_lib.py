@@ -35,7 +35,7 @@ class ControlArgs(C.Structure):
 
 class RabiArgs(C.Structure):
     _fields_ = [("ddt", C.c_double), ("nsteps", C.c_int), ("checkpoint_blocks", C.c_int), ("omega0", C.c_double),
-                ("k", C.c_double)]
+                ("k", C.c_double), ("tape_gib", C.c_double)]
 
 
 class ItpArgs(C.Structure):
@@ -266,7 +266,7 @@ class Device:
         return loss.value, grad.reshape(np.asarray(coeffs).shape, order="F"), rerr.value
 
     def rabi_forward(self, ddt, nsteps, omega0, k, ckpt=0, want_state=True):
-        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k))
+        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k), 0.0)
         loss = C.c_double()
         if not want_state:
             self.call("gpe_rabi_forward", C.byref(a), C.byref(loss), None)
@@ -276,8 +276,8 @@ class Device:
         return loss.value, [from_colmajor_c128(out[:self.npts], self.shape),
                             from_colmajor_c128(out[self.npts:], self.shape)]
 
-    def rabi_grad(self, ddt, nsteps, omega0, k, ckpt=0):
-        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k))
+    def rabi_grad(self, ddt, nsteps, omega0, k, ckpt=0, tape_gib=0.0):
+        a = RabiArgs(float(ddt), int(nsteps), int(ckpt), float(omega0), float(k), float(tape_gib))
         loss = C.c_double()
         grad = np.zeros(2)
         self.call("gpe_rabi_grad", C.byref(a), C.byref(loss), dptr(grad))

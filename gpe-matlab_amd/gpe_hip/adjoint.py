@@ -84,10 +84,12 @@ class RabiControl:
     The run starts from ``task.init_state`` with ``task.g``, ``task.Vtrap`` (static
     potentials) and ``task.n_recalc``; ``nsteps`` must be a multiple of n_recalc.
     The two-component step is not explicitly invertible, so the backward pass
-    recomputes checkpoint segments of about sqrt(nsteps) steps from stored states.
+    reads the state before every step from a tape: segments of about sqrt(nsteps)
+    steps are stored by the forward pass while ``tape_gib`` (0: half the free device
+    memory, < 0: one segment) holds them, and recomputed from checkpoints otherwise.
     """
 
-    def __init__(self, task, ddt, nsteps, checkpoint_blocks=0):
+    def __init__(self, task, ddt, nsteps, checkpoint_blocks=0, tape_gib=0.0):
         if task.gamma != 0:
             raise ValueError("the Rabi adjoint needs gamma = 0")
         if not task.Ntotal > 0:
@@ -98,6 +100,7 @@ class RabiControl:
         self.ddt = float(ddt)
         self.nsteps = int(nsteps)
         self.ckpt = int(checkpoint_blocks)
+        self.tape_gib = float(tape_gib)
 
     def _prepare(self):
         t = self.task
@@ -118,7 +121,7 @@ class RabiControl:
 
     def loss_and_grad(self, omega0, k):
         """(L, array([dL/dOmega0, dL/dk]))"""
-        return self._prepare().rabi_grad(self.ddt, self.nsteps, omega0, k, self.ckpt)
+        return self._prepare().rabi_grad(self.ddt, self.nsteps, omega0, k, self.ckpt, self.tape_gib)
 
     def autograd_function(self):
         """A torch.autograd.Function mapping a float64 tensor (Omega0, k) to L."""

@@ -145,14 +145,18 @@ int gpe_adjoint_grad(gpe_handle* h, const gpe_control_args* a, const double* coe
  * population imbalance (N1 - N2) / Ntotal.  Both calls start from the uploaded
  * two-component state and leave it in place; they leave the handle's coupling set
  * to that Omega(x), and a later solve_split starts with gpe_split_begin.
- * The gradient recomputes checkpoint segments (about sqrt(T) steps each) and
- * needs about (nsteps / segments + 2 * segments + 2) state-sized arrays of HBM. */
+ * The gradient cuts the run into segments (about sqrt(T) steps each).  It stores
+ * the state before every step of as many trailing segments as tape_gib holds and
+ * recomputes the others from per-segment checkpoints; device memory: about
+ * 2 * (segments + 1) state arrays plus the tape (2 state arrays per stored step). */
 typedef struct {
   double ddt;
-  int nsteps;             /* a multiple of n_recalc                         */
-  int checkpoint_blocks;  /* blocks per segment, 0: about sqrt(nsteps) steps */
+  int nsteps;             /* a multiple of n_recalc                               */
+  int checkpoint_blocks;  /* blocks per segment, 0: about sqrt(nsteps) steps       */
   double omega0;
   double k;
+  double tape_gib;        /* tape budget in GiB; 0: half the free device memory, < 0:
+                             one segment (every other segment is recomputed)         */
 } gpe_rabi_args;
 /* psi_out (may be NULL): 2 * npts complex values, component 1 then component 2 */
 int gpe_rabi_forward(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* psi_out);

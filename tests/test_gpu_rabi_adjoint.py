@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 G_MAT = [[0.5, 0.45], [0.4, 0.6]]   # asymmetric, so a transposed index shows
 
 
-def problem(dims, n, xmax, nsteps, ddt=0.004, seed=4, ckpt=0):
+def problem(dims, n, xmax, nsteps, ddt=0.004, seed=4, ckpt=0, tape_gib=0.0):
     import gpe_hip
     from gpe_hip.adjoint import RabiControl
     if dims == 2:
@@ -32,7 +32,7 @@ def problem(dims, n, xmax, nsteps, ddt=0.004, seed=4, ckpt=0):
     task.g = np.array(G_MAT)
     task.Ntotal = N
     task.init_state = [a.copy(), b.copy()]
-    return ref, RabiControl(task, ddt, nsteps, checkpoint_blocks=ckpt), task
+    return ref, RabiControl(task, ddt, nsteps, checkpoint_blocks=ckpt, tape_gib=tape_gib), task
 
 
 CASES = [(2, (32, 24), (6, 5)), (2, (64, 64), (7, 7)), (3, (16, 24, 12), (5, 6, 4))]
@@ -73,10 +73,13 @@ def test_rabi_grad_matches_central_fd(om0, k):
     assert abs(g[1] - fdk) <= 1e-6 * abs(fdk) + 1e-9
 
 
-def test_rabi_checkpoint_segments_do_not_change_the_gradient():
+@pytest.mark.parametrize("tape_gib", [-1.0, 0.0, 3 * 10 * 2 * 32 * 24 * 16 / 2 ** 30])
+def test_rabi_checkpoint_segments_do_not_change_the_gradient(tape_gib):
+    # tape_gib: one segment stored (all others recomputed), half the free memory
+    # (everything stored), and room for exactly 3 of the 6 blocks of 10 steps
     grads = []
     for ckpt in (0, 1, 2, 3, 6):
-        _, ctl, _ = problem(2, (32, 24), (6, 5), 60, ckpt=ckpt)
+        _, ctl, _ = problem(2, (32, 24), (6, 5), 60, ckpt=ckpt, tape_gib=tape_gib)
         grads.append(ctl.loss_and_grad(0.6, 1.1))
     for L, g in grads[1:]:
         assert L == grads[0][0]
