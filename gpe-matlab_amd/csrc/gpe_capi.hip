@@ -83,6 +83,11 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     const long long nb = (h->ax[a].ncols + C - 1) / C;
     if (nb > maxwg) maxwg = (int)nb;
   }
+  {  // two-component x-sweeps put C/2 columns in a work-group (one lane per component)
+    const int C2 = gpe::sweep_cols_per_block(nn[0]) / 2;
+    const long long nb = (h->ax[0].ncols + C2 - 1) / C2;
+    if (nb > maxwg) maxwg = (int)nb;
+  }
   h->maxwg = maxwg;
 
   auto bail = [&](int code, const char* what) {
@@ -236,6 +241,12 @@ int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
   if (is_array) {
     if (!h->cpl) HIPCHK(h, hipMalloc(&h->cpl, h->npts * sizeof(double2)));
     HIPCHK(h, hipMemcpyAsync(h->cpl, omega, h->npts * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    double m2 = 0.0;
+    for (long long i = 0; i < h->npts; ++i) {
+      const double a2 = omega[2 * i] * omega[2 * i] + omega[2 * i + 1] * omega[2 * i + 1];
+      if (!(a2 <= m2)) m2 = a2;
+    }
+    h->cpl_max2 = m2;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cpl_array = true;
   } else {

@@ -46,6 +46,7 @@ struct gpe_handle {
   double2* cpl = nullptr;       // spinor coupling array (or null for scalar)
   double cpl_re = 0, cpl_im = 0;
   bool cpl_array = false;
+  double cpl_max2 = 0;          // max |Omega|^2 of the coupling array (NaN propagates)
   double2* kcol = nullptr;      // kinetic factor per mid-axis column
   double2* kax = nullptr;       // kinetic factor per mid-axis index (incl. 1/npts)
   double2* kcol_sp[2] = {nullptr, nullptr};  // spinor: exp(-kk dt/2) and exp(-kk dt) exp(kk dt/2)

@@ -187,6 +187,7 @@ int begin(gpe_handle* h, const gpe_rabi_args* a) {
                      h->rb_xc, a->omega0, a->k, (long long)h->ny, (long long)h->nx, (long long)h->npts);
   HIPCHK(h, hipGetLastError());
   h->cpl_array = true;
+  h->cpl_max2 = a->omega0 * a->omega0;
   if ((rc = gpe::spinor_split_begin(h, a->ddt, h->prm.n_recalc, nullptr))) return rc;
   rabi_scalars(h);
   return GPE_OK;

@@ -24,6 +24,7 @@ gpe::SpinorParams spinor_params(gpe_handle* h) {
   p.V1 = h->V[0];
   p.V2 = h->V[1];
   p.cpl = h->cpl_array ? h->cpl : nullptr;
+  p.cpl_max2 = h->cpl_max2;
   p.tw = a.tw;
   p.partials = h->partials;
   p.st = h->st;

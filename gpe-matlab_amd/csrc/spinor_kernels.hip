@@ -1,5 +1,11 @@
 // Two-component x-sweeps of GPEtaskSpinor for one FFT length (-DGPE_N=<n>).
 //
+// Lane pairs: lane 2i holds component 1 and lane 2i+1 component 2 of the same
+// pencil positions, so each lane transforms one component (E values, the
+// register footprint of a one-component sweep) and the pointwise 2x2 step reads
+// the other component from its partner lane with a DPP swap instead of holding
+// both.  The two components' FFTs run side by side in two halves of LDS.
+//
 // SP_PR : inverse x-FFT of both components, the fused potential/interaction
 //         phase and 2x2 Rabi rotation (lib/@GPEtaskSpinor/solve_split.m:77-91),
 //         forward x-FFT of both components.
@@ -32,147 +38,200 @@ __device__ __forceinline__ void ccoshsinh(double x, double y, double2* ch, doubl
   *sh = make_double2(shx * c, chx * s);
 }
 
-template <int N, int C, int SEQ>
+// cos(h r) and sin(h r) / r from z = (h r)^2, r = |Omega| (the Rabi factors for
+// dt = i h): Taylor series in z up to (h r)^16, truncation below 1e-17 for z <= 0.6.
+__device__ __forceinline__ void rabi_taylor(double h, double z, double* cs, double* f) {
+  double c = 1.0 / 20922789888000.0;                  //  1/16!
+  c = fma(c, z, -1.0 / 87178291200.0);                // -1/14!
+  c = fma(c, z, 1.0 / 479001600.0);                   //  1/12!
+  c = fma(c, z, -1.0 / 3628800.0);                    // -1/10!
+  c = fma(c, z, 1.0 / 40320.0);                       //  1/8!
+  c = fma(c, z, -1.0 / 720.0);                        // -1/6!
+  c = fma(c, z, 1.0 / 24.0);                          //  1/4!
+  c = fma(c, z, -0.5);                                // -1/2!
+  *cs = fma(c, z, 1.0);
+  double s = 1.0 / 355687428096000.0;                 //  1/17!
+  s = fma(s, z, -1.0 / 1307674368000.0);              // -1/15!
+  s = fma(s, z, 1.0 / 6227020800.0);                  //  1/13!
+  s = fma(s, z, -1.0 / 39916800.0);                   // -1/11!
+  s = fma(s, z, 1.0 / 362880.0);                      //  1/9!
+  s = fma(s, z, -1.0 / 5040.0);                       // -1/7!
+  s = fma(s, z, 1.0 / 120.0);                         //  1/5!
+  s = fma(s, z, -1.0 / 6.0);                          // -1/3!
+  *f = h * fma(s, z, 1.0);
+}
+
+// The value of the partner lane (lane ^ 1): DPP quad_perm [1, 0, 3, 2].
+__device__ __forceinline__ double xchg(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double2 xchg(double2 v) { return make_double2(xchg(v.x), xchg(v.y)); }
+
+// Coupling kinds of SP_PR, fixed per launch.
+// CK_ARRAY_REAL: array Omega with dt = i di; CK_ARRAY_TAYLOR: the same with
+// di^2 max|Omega|^2 <= 0.6 (rabi_maxz), where the Rabi factors are power series.
+// CK_SCALAR_REAL: scalar Omega with dt = i di.
+enum CplKind : int {
+  CK_SCALAR = 0, CK_ARRAY = 1, CK_NONLIN = 2, CK_ARRAY_REAL = 3, CK_ARRAY_TAYLOR = 4, CK_SCALAR_REAL = 5
+};
+
+template <int N, int C2, int SEQ, int CK>
 __global__ void __launch_bounds__(256) spinor_kernel(const SpinorParams p) {
   using PL = Plan<N>;
+  using L = Lds<N, C2, true>;
   constexpr int T = PL::T, E = PL::E;
   constexpr bool COLFAST = true;
-  __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
+  __shared__ double lds[2 * L::SIZE];
   __shared__ double2 twl[N];
+  __shared__ double2 exl[SEQ == SP_ADJ ? N : 1];   // SP_ADJ: exp(i kw x) per x index
+  __shared__ double xl[SEQ == SP_ADJ ? N : 1];     // SP_ADJ: x per x index
   __shared__ double red[4];
 
   const int tid = threadIdx.x;
-  const int c = tid % C;
-  const int t = tid / C;
+  const int k = tid & 1;           // component of this lane
+  const int q = tid >> 1;
+  const int c = q % C2;
+  const int t = q / C2;
+  double* ldsk = lds + k * L::SIZE;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
+  if constexpr (SEQ == SP_ADJ) {
+    for (int i = tid; i < N; i += 256) {
+      double ey, ex;
+      sincos_cw(p.kw * p.xc[i], &ey, &ex);
+      exl[i] = make_double2(ex, ey);
+      xl[i] = p.xc[i];
+    }
+    __syncthreads();
+  }
 
-  const unsigned cgu = blockIdx.x * (unsigned)C + (unsigned)c;
-  const bool active = cgu < p.ncols;
+  const unsigned cgu = blockIdx.x * (unsigned)C2 + (unsigned)c;
+  const bool active = cgu < p.ncols;   // equal on both lanes of a pair
   const unsigned cg = active ? cgu : p.ncols - 1u;
   const unsigned base = (cg % p.A) * p.sa + (cg / p.A) * p.sb + (unsigned)t * p.stride;
   const unsigned step = (unsigned)T * p.stride;
+  double2* __restrict__ psik = k ? p.psi2 : p.psi1;
+  const double* __restrict__ Vk = k ? p.V2 : p.V1;
 
-  double2 u[E], w[E];
+  double2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    u[m] = p.psi1[base + (unsigned)m * step];
-    w[m] = p.psi2[base + (unsigned)m * step];
-  }
+  for (int m = 0; m < E; ++m) v[m] = psik[base + (unsigned)m * step];
 
   if constexpr (SEQ == SP_PR) {
-    fft_pencil<PL, +1, C, COLFAST>(u, t, c, lds, twl);
-    fft_pencil<PL, +1, C, COLFAST>(w, t, c, lds, twl);
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
     if (p.ust1 && active) {   // keep the pre-step state for the adjoint's segment recomputation
+      double2* __restrict__ us = k ? p.ust2 : p.ust1;
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        p.ust1[base + (unsigned)m * step] = u[m];
-        p.ust2[base + (unsigned)m * step] = w[m];
-      }
+      for (int m = 0; m < E; ++m) us[base + (unsigned)m * step] = v[m];
     }
     const double mu = p.st->mu;
-    const bool has_re = p.dr != 0.0;
+    constexpr bool real_dt = CK == CK_ARRAY_REAL || CK == CK_ARRAY_TAYLOR || CK == CK_SCALAR_REAL;
+    const bool has_re = !real_dt && p.dr != 0.0;
+    const double sgn = k ? 1.0 : -1.0;   // row 1: i f Omega, row 2: i f conj(Omega)
+    // this lane's row of the interaction matrix: a_k = V_k - mu + g_k1 n1 + g_k2 n2
+    const double gk1 = k ? p.g21 : p.g11, gk2 = k ? p.g22 : p.g12;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const unsigned idx = base + (unsigned)m * step;
-      const double n1 = cabs2(u[m]), n2 = cabs2(w[m]);
-      const double a1 = ((p.V1[idx] - mu) + p.g11 * n1) + p.g12 * n2;   // solve_split.m:78-83
-      const double a2 = ((p.V2[idx] - mu) + p.g21 * n1) + p.g22 * n2;
-      double2 cosom, sinomm, sinomp;
-      if (p.nlincpl != 0.0) {                                           // solve_split.m:84-89
+      const double2 o = xchg(v[m]);
+      const double nown = cabs2(v[m]), noth = cabs2(o);
+      const double n1 = k ? noth : nown, n2 = k ? nown : noth;
+      const double a = ((Vk[idx] - mu) + gk1 * n1) + gk2 * n2;   // solve_split.m:78-83
+      double2 cosom, soff;   // diagonal and this row's off-diagonal Rabi factor
+      if constexpr (CK == CK_NONLIN) {                            // solve_split.m:84-89
         const double2 om = p.cpl ? p.cpl[idx] : p.cpl_s;
         const double2 t3 = make_double2(om.x + p.nlincpl * (n1 + n2), om.y);
         const double2 z = cmul(make_double2(p.dr, p.di), t3);
-        ccoshsinh(z.x, z.y, &cosom, &sinomm);
-        sinomp = sinomm;
-      } else if (p.cpl) {                                               // solve_split.m:61-65, array Omega
+        ccoshsinh(z.x, z.y, &cosom, &soff);
+      } else if constexpr (CK == CK_ARRAY) {                      // solve_split.m:61-65, array Omega
         const double2 om = p.cpl[idx];
         const double r = hypot(om.x, om.y);
         double2 sh;
         ccoshsinh(p.dr * r, p.di * r, &cosom, &sh);
         const double2 ph = r > 0.0 ? make_double2(om.x / r, om.y / r) : make_double2(1.0, 0.0);   // e^{i angle}
-        sinomm = cmulc(sh, ph);
-        sinomp = cmul(sh, ph);
+        soff = k ? cmulc(sh, ph) : cmul(sh, ph);
+      } else if constexpr (CK == CK_ARRAY_REAL || CK == CK_ARRAY_TAYLOR) {   // the same for dt = i di:
+        const double2 om = p.cpl[idx];          // cosh(dt r) = cos(di r), sinh(dt r) = i sin(di r)
+        double cs, f;                           // sin(di r) e^{+-i angle} = f Omega or f conj(Omega)
+        if constexpr (CK == CK_ARRAY_TAYLOR) {
+          rabi_taylor(p.di, p.di * p.di * (om.x * om.x + om.y * om.y), &cs, &f);
+        } else {
+          const double r = sqrt(om.x * om.x + om.y * om.y);
+          double sn;
+          sincos_cw(p.di * r, &sn, &cs);
+          f = r > 0.0 ? sn / r : p.di;
+        }
+        cosom = make_double2(cs, 0.0);
+        soff = make_double2(sgn * f * om.y, f * om.x);
       } else {
         cosom = p.cosom;
-        sinomm = p.sinomm;
-        sinomp = p.sinomp;
+        soff = k ? p.sinomm : p.sinomp;
       }
-      const double2 r1 = csub(cmul(cosom, u[m]), cmul(sinomp, w[m]));   // solve_split.m:90
-      const double2 r2 = csub(cmul(cosom, w[m]), cmul(sinomm, u[m]));   // solve_split.m:91
-      u[m] = mul_cexp(r1, -a1 * p.dr, -a1 * p.di, has_re);
-      w[m] = mul_cexp(r2, -a2 * p.dr, -a2 * p.di, has_re);
+      const double2 r = csub(cmul(cosom, v[m]), cmul(soff, o));   // solve_split.m:90-91
+      v[m] = mul_cexp(r, -a * p.dr, -a * p.di, has_re);
     }
-    fft_pencil<PL, -1, C, COLFAST>(u, t, c, lds, twl);
-    fft_pencil<PL, -1, C, COLFAST>(w, t, c, lds, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        p.psi1[base + (unsigned)m * step] = u[m];
-        p.psi2[base + (unsigned)m * step] = w[m];
-      }
+      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ADJ) {
-    // u, w hold the adjoint components (x in k space); ust1/ust2 hold the pre-step state
-    fft_pencil<PL, +1, C, COLFAST>(u, t, c, lds, twl);
-    fft_pencil<PL, +1, C, COLFAST>(w, t, c, lds, twl);
+    // v holds this lane's adjoint component (x in k space); ust1/ust2 hold the
+    // pre-step state
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    const double2* __restrict__ usk = k ? p.ust2 : p.ust1;
     const double mu = p.st->mu;
     const double h = p.di;   // gamma = 0: dt = i h
     double sn, cs;
     sincos_cw(h * p.om0, &sn, &cs);
+    const double gk1 = k ? p.g21 : p.g11, gk2 = k ? p.g22 : p.g12;   // a_k = V_k - mu + g_k1 n1 + g_k2 n2
+    const double hq1 = h * (k ? p.g12 : p.g11), hq2 = h * (k ? p.g22 : p.g21);   // A_k = h (g_1k q1 + g_2k q2)
     double g0 = 0.0, gk = 0.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const unsigned idx = base + (unsigned)m * step;
-      const double2 s1 = p.ust1[idx], s2 = p.ust2[idx];
-      const double n1 = cabs2(s1), n2 = cabs2(s2);
-      const double a1 = ((p.V1[idx] - mu) + p.g11 * n1) + p.g12 * n2;
-      const double a2 = ((p.V2[idx] - mu) + p.g21 * n1) + p.g22 * n2;
+      const double2 sk = usk[idx];
+      const double2 so = xchg(sk);
+      const double nown = cabs2(sk), noth = cabs2(so);
+      const double n1 = k ? noth : nown, n2 = k ? nown : noth;
+      const double a = ((Vk[idx] - mu) + gk1 * n1) + gk2 * n2;
       // R = [[c, -i s e], [-i s conj(e), c]] with c = cos(h om0), s = sin(h om0),
       // e = exp(i kw X) (solve_split.m:61-65, 90-91 for dt = i h)
-      const double X = p.xc[t + T * m];
-      double ex, ey;
-      sincos_cw(p.kw * X, &ey, &ex);
-      const double2 sinomp = make_double2(-sn * ey, sn * ex);    // i s e
-      const double2 sinomm = make_double2(sn * ey, sn * ex);     // i s conj(e)
-      const double2 cosom = make_double2(cs, 0.0);
-      // forward map of this point: y = R s, w = e^{-i theta} y
-      const double2 y1 = csub(cmul(cosom, s1), cmul(sinomp, s2));
-      const double2 y2 = csub(cmul(cosom, s2), cmul(sinomm, s1));
-      const double2 w1 = mul_cexp(y1, 0.0, -a1 * h, false);
-      const double2 w2 = mul_cexp(y2, 0.0, -a2 * h, false);
-      const double2 l1 = u[m], l2 = w[m];
-      const double2 m1 = mul_cexp(l1, 0.0, a1 * h, false);   // e^{i theta_1} lambda_1
-      const double2 m2 = mul_cexp(l2, 0.0, a2 * h, false);
-      const double q1 = l1.x * w1.y - l1.y * w1.x;           // Im(conj(lambda) w)
-      const double q2 = l2.x * w2.y - l2.y * w2.x;
-      const double A1 = h * (p.g11 * q1 + p.g21 * q2);
-      const double A2 = h * (p.g12 * q1 + p.g22 * q2);
-      // lambda_u = R^H m + 2 A s
-      const double2 nm = make_double2(-sinomm.x, -sinomm.y), np_ = make_double2(-sinomp.x, -sinomp.y);
-      const double2 lu1 = cadd(cadd(cmulc(m1, cosom), cmulc(m2, nm)), cscale(s1, 2.0 * A1));
-      const double2 lu2 = cadd(cadd(cmulc(m1, np_), cmulc(m2, cosom)), cscale(s2, 2.0 * A2));
-      // dL/dp += Re m^H (dR/dp) s:  dR/dom0 = [[-h s, -i h c e], [-i h c conj(e), -h s]],
+      const double X = xl[t + T * m];
+      const double ex = exl[t + T * m].x, ey = exl[t + T * m].y;
+      const double2 sp = make_double2(-sn * ey, sn * ex);   // i s e
+      const double2 sm = make_double2(sn * ey, sn * ex);    // i s conj(e)
+      const double2 srow = k ? sm : sp;                     // R_{k,other} = -srow
+      const double2 scol = k ? sp : sm;                     // R_{other,k} = -scol
+      // forward map of this point: y_k = c s_k - srow s_other, w_k = e^{-i theta_k} y_k
+      const double2 y = csub(cscale(sk, cs), cmul(srow, so));
+      double sth, cth;
+      sincos_cw(a * h, &sth, &cth);
+      const double2 wk = cmul(y, make_double2(cth, -sth));
+      const double2 lk = v[m];
+      const double2 mk = cmul(lk, make_double2(cth, sth));   // e^{i theta_k} lambda_k
+      const double qk = lk.x * wk.y - lk.y * wk.x;          // Im(conj(lambda_k) w_k)
+      const double qo = xchg(qk);
+      const double2 mo = xchg(mk);
+      const double A = k ? hq1 * qo + hq2 * qk : hq1 * qk + hq2 * qo;
+      // lambda_u,k = conj(c) m_k + conj(-scol) m_other + 2 A_k s_k
+      const double2 nsc = make_double2(-scol.x, -scol.y);
+      v[m] = cadd(cadd(cscale(mk, cs), cmulc(mo, nsc)), cscale(sk, 2.0 * A));
+      // dL/dp += Re conj(m_k) (dR/dp s)_k:  dR/dom0 = [[-h s, -i h c e], [-i h c conj(e), -h s]],
       // dR/dkw = [[0, s X e], [-s X conj(e), 0]]
-      const double2 d12 = make_double2(h * cs * ey, -h * cs * ex);
-      const double2 d21 = make_double2(-h * cs * ey, -h * cs * ex);
-      const double2 t01 = cadd(cscale(s1, -h * sn), cmul(d12, s2));
-      const double2 t02 = cadd(cmul(d21, s1), cscale(s2, -h * sn));
-      g0 += (m1.x * t01.x + m1.y * t01.y) + (m2.x * t02.x + m2.y * t02.y);
-      const double2 k12 = make_double2(sn * X * ex, sn * X * ey);
-      const double2 k21 = make_double2(-sn * X * ex, sn * X * ey);
-      const double2 tk1 = cmul(k12, s2), tk2 = cmul(k21, s1);
-      gk += (m1.x * tk1.x + m1.y * tk1.y) + (m2.x * tk2.x + m2.y * tk2.y);
-      u[m] = lu1;
-      w[m] = lu2;
+      const double2 dk = k ? make_double2(-h * cs * ey, -h * cs * ex) : make_double2(h * cs * ey, -h * cs * ex);
+      const double2 t0 = cadd(cscale(sk, -h * sn), cmul(dk, so));
+      g0 += mk.x * t0.x + mk.y * t0.y;
+      const double2 kk = k ? make_double2(-sn * X * ex, sn * X * ey) : make_double2(sn * X * ex, sn * X * ey);
+      const double2 tk = cmul(kk, so);
+      gk += mk.x * tk.x + mk.y * tk.y;
     }
-    fft_pencil<PL, -1, C, COLFAST>(u, t, c, lds, twl);
-    fft_pencil<PL, -1, C, COLFAST>(w, t, c, lds, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        p.psi1[base + (unsigned)m * step] = u[m];
-        p.psi2[base + (unsigned)m * step] = w[m];
-      }
+      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
     }
     if (!active) {
       g0 = 0.0;
@@ -186,36 +245,35 @@ __global__ void __launch_bounds__(256) spinor_kernel(const SpinorParams p) {
     }
   } else {
     double accA = 0.0, accB = 0.0, accC1 = 0.0, accC12 = 0.0, accC2 = 0.0, accX = 0.0, k1 = 0.0, k2 = 0.0;
-    if (active) {
+    const double own1 = k ? 0.0 : 1.0;   // sums that belong to component 1's lane only
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const unsigned idx = base + (unsigned)m * step;
-        const double n1 = cabs2(u[m]), n2 = cabs2(w[m]);
-        const double2 om = p.cpl ? p.cpl[idx] : p.cpl_s;
-        accA += n1 + n2;
-        accB += p.V1[idx] * n1 + p.V2[idx] * n2;
-        accC1 += n1 * n1;
-        accC12 += n1 * n2;
-        accC2 += n2 * n2;
-        const double2 ow = cmul(om, w[m]);
-        accX += u[m].x * ow.x + u[m].y * ow.y;   // Re conj(psi1) Omega psi2
-      }
+    for (int m = 0; m < E; ++m) {
+      const unsigned idx = base + (unsigned)m * step;
+      const double2 o = xchg(v[m]);
+      const double nown = cabs2(v[m]), noth = cabs2(o);
+      const double2 om = p.cpl ? p.cpl[idx] : p.cpl_s;
+      accA += nown;
+      accB += Vk[idx] * nown;
+      accC1 += own1 * (nown * nown);
+      accC12 += own1 * (nown * noth);
+      accC2 += (1.0 - own1) * (nown * nown);
+      const double2 ow = cmul(om, o);
+      accX += own1 * (v[m].x * ow.x + v[m].y * ow.y);   // Re conj(psi1) Omega psi2
     }
-    fft_pencil<PL, -1, C, COLFAST>(u, t, c, lds, twl);
-    fft_pencil<PL, -1, C, COLFAST>(w, t, c, lds, twl);
-    if (active) {
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        k1 += p.keax[t + T * m] * cabs2(u[m]);
-        k2 += p.keax[t + T * m] * cabs2(w[m]);
-      }
+    for (int m = 0; m < E; ++m) {
+      const double e = p.keax[t + T * m] * cabs2(v[m]);
+      k1 += own1 * e;
+      k2 += (1.0 - own1) * e;
     }
+    if (!active) accA = accB = accC1 = accC12 = accC2 = accX = k1 = k2 = 0.0;
     const double vals[8] = {accA, accB, accC1, accC12, accC2, accX, k1, k2};
     const int slots[8] = {SL_A, SL_B, SL_C, SL_C12, SL_C2, SL_X, SL_KX, SL_KX2};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const double r = block_sum(vals[q], red);
-      if (tid == 0) p.partials[slots[q] * p.maxwg + blockIdx.x] = r;
+    for (int i = 0; i < 8; ++i) {
+      const double r = block_sum(vals[i], red);
+      if (tid == 0) p.partials[slots[i] * p.maxwg + blockIdx.x] = r;
     }
   }
 }
@@ -225,18 +283,31 @@ __global__ void __launch_bounds__(256) spinor_kernel(const SpinorParams p) {
 
 hipError_t GPE_CAT(launch_spinor_sweep_, GPE_N)(int seq, const SpinorParams& p, hipStream_t s, int* nblocks) {
   constexpr int N = GPE_N;
-  constexpr int C = 256 / Plan<N>::T;
-  const long long nb = ((long long)p.ncols + C - 1) / C;
+  constexpr int C2 = 128 / Plan<N>::T;   // columns per block, two lanes per column position
+  const long long nb = ((long long)p.ncols + C2 - 1) / C2;
   if (nblocks) *nblocks = (int)nb;
   if (nb <= 0) return hipSuccess;
-  if (seq == SP_PR)
-    hipLaunchKernelGGL((spinor_kernel<N, C, SP_PR>), dim3((unsigned)nb), dim3(256), 0, s, p);
-  else if (seq == SP_KEX)
-    hipLaunchKernelGGL((spinor_kernel<N, C, SP_KEX>), dim3((unsigned)nb), dim3(256), 0, s, p);
-  else if (seq == SP_ADJ)
-    hipLaunchKernelGGL((spinor_kernel<N, C, SP_ADJ>), dim3((unsigned)nb), dim3(256), 0, s, p);
-  else
+  const dim3 g((unsigned)nb), b(256);
+  if (seq == SP_PR) {
+    if (p.nlincpl != 0.0)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_NONLIN>), g, b, 0, s, p);
+    else if (p.cpl && p.dr == 0.0 && p.di * p.di * p.cpl_max2 <= 0.6)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_ARRAY_TAYLOR>), g, b, 0, s, p);
+    else if (p.cpl && p.dr == 0.0)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_ARRAY_REAL>), g, b, 0, s, p);
+    else if (p.cpl)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_ARRAY>), g, b, 0, s, p);
+    else if (p.dr == 0.0)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_SCALAR_REAL>), g, b, 0, s, p);
+    else
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_SCALAR>), g, b, 0, s, p);
+  } else if (seq == SP_KEX) {
+    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEX, CK_SCALAR>), g, b, 0, s, p);
+  } else if (seq == SP_ADJ) {
+    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ADJ, CK_SCALAR>), g, b, 0, s, p);
+  } else {
     return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -77,6 +77,7 @@ struct SpinorParams {
   const double* V1;
   const double* V2;
   const double2* cpl;        // space-dependent coupling Omega(x), or null for a scalar
+  double cpl_max2;           // max |Omega(x)|^2 over the array (selects the series Rabi factors)
   const double2* tw;
   double* partials;
   DevState* st;
