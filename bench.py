@@ -480,6 +480,15 @@ def bench_c5_adjoint(t, log, omega0=0.5, k=2.0, nsteps=1000, ddt=0.002, h=1e-6):
     Lg, g = dev.rabi_grad(ddt, nsteps, omega0, k)
     eg = time.perf_counter() - t0
     log(f"c5 adjoint: forward {ef:.3f}s, gradient {eg:.3f}s (ratio {eg / ef:.2f}); L={L:.12f} grad={g}")
+    # per-sweep split of one gradient (HIP events around every launch; spinor x-sweeps are
+    # reported as mid_x = phase/Rabi step, kex_x = block-end sums, close_x = transposed step)
+    from gpe_hip._lib import AXIS_NAMES, SEQ_NAMES
+    dev.profile_enable(True)
+    dev.rabi_grad(ddt, nsteps, omega0, k)
+    prof = dev.profile_read()
+    dev.profile_enable(False)
+    split = {f"{SEQ_NAMES[q]}_{AXIS_NAMES[a]}": [round(ms, 2), n] for (q, a), (ms, n) in sorted(prof.items())}
+    log(f"c5 gradient sweeps (total ms, launches): {split}")
     fd = []
     for i in range(2):
         p = [omega0, k]
@@ -505,7 +514,8 @@ def bench_c5_adjoint(t, log, omega0=0.5, k=2.0, nsteps=1000, ddt=0.002, h=1e-6):
     return dict(adjoint_loss=L, adjoint_grad=[float(x) for x in g], adjoint_fd=fd, adjoint_fd_rel=rel,
                 adjoint_fd4=fd4, adjoint_fd4_H=H, adjoint_fd4_rel=rel4,
                 adjoint_forward_s=round(ef, 4), adjoint_grad_s=round(eg, 4),
-                adjoint_over_forward=round(eg / ef, 3), adjoint_same_loss=bool(L == Lg))
+                adjoint_over_forward=round(eg / ef, 3), adjoint_same_loss=bool(L == Lg),
+                adjoint_sweeps_ms=split)
 
 
 if __name__ == "__main__":
