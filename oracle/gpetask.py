@@ -454,3 +454,194 @@ class GPEtaskSpinor(GPEtask):
                     mu = self.inner(phi, hphi) / NNN
             self.ext_callback(list(phi), j, time2, mu, ncur)     # 117
         return phi
+
+    def _itp_init(self, phi, ncomp):
+        # GPEtaskSpinor/groundstate_itp.m:26-28 (rand, seeded here)
+        if phi is None:
+            g = self.grid
+            phi = []
+            for _ in range(ncomp):
+                r = self.rng.random(g.size) + 1j * self.rng.random(g.size)
+                phi.append(g.normalize(r) / np.sqrt(ncomp))
+        return [p.astype(np.complex128, copy=True) for p in phi]
+
+    def groundstate_itp(self, dt, eps, phi=None, nargout=1, max_iter=500000):
+        """@GPEtaskSpinor/groundstate_itp.m:1-145 (omega = 0): K/2 . (P R) . K/2 with the
+        density taken before the first kinetic half step, total-norm renormalisation
+        to Ntotal, and a step-size cascade dt -> dt/1.5 on the energy-like series EE."""
+        self._check_omega()
+        grid = self.grid
+        ncomp = np.asarray(self.g).shape[0]                      # 20-21
+        self.ncomp = ncomp
+        VV = self.getVtotal(0)                                   # 22
+        coupl = self.coupling                                    # 24
+        g = np.asarray(self.g)
+        phi = self._itp_init(phi, ncomp)                         # 26-28
+        ekk = np.exp(-grid.kk * dt * 0.5)                        # 29
+        MU, MU2, EE = [], [], []
+        i = 1                                                    # 39
+        iswitch = 20                                             # 40
+        tmp2 = [None] * ncomp
+        cang = np.angle(coupl)                                   # 43-46
+        cosom = np.cosh(dt * np.abs(coupl))
+        sinomm = np.sinh(dt * np.abs(coupl)) * np.exp(-1j * cang)
+        sinomp = np.sinh(dt * np.abs(coupl)) * np.exp(1j * cang)
+        converged = True
+        while True:
+            for j in range(ncomp):                               # 49-54
+                tmp2[j] = VV[j]
+                for k in range(ncomp):
+                    tmp2[j] = tmp2[j] + g[j, k] * np.abs(phi[k]) ** 2
+            phi[0] = grid.ifft(ekk * grid.fft(phi[0]))           # 67-68
+            phi[1] = grid.ifft(ekk * grid.fft(phi[1]))
+            t0 = np.exp(-tmp2[0] * dt) * (cosom * phi[0] - sinomp * phi[1])   # 70
+            t1 = np.exp(-tmp2[1] * dt) * (cosom * phi[1] - sinomm * phi[0])   # 71
+            phi[0] = grid.ifft(ekk * grid.fft(t0))               # 78-79
+            phi[1] = grid.ifft(ekk * grid.fft(t1))
+            ntot = np.real(grid.integrate(phi[0] * np.conj(phi[0]) + phi[1] * np.conj(phi[1])))   # 82
+            mu = np.sqrt(self.Ntotal / ntot)                     # 83
+            MU.append(np.log(mu) / dt)                           # 84
+            for j in range(ncomp):                               # 86-88
+                phi[j] = phi[j] * mu
+            self.current_state = phi
+            if nargout >= 3:                                     # 91-95
+                MU2.append(np.real(self.inner(phi, self.applyham(phi))) / self.Ntotal)
+            else:
+                MU2.append(MU[-1])
+            if nargout >= 4:                                     # 96-100
+                EE.append(self.get_energy(phi) / self.Ntotal)
+            else:
+                EE.append(MU2[-1])
+            if (i - iswitch) > 10 and i % 10 == 0:               # 102-123
+                delta = (abs(EE[i - 1] - EE[i - 10]) / 9 + abs(EE[i - 1] - EE[i - 2])) / dt
+                if delta < eps:
+                    if dt < eps or dt < 5e-6:
+                        break
+                    dt = dt / 1.5
+                    ekk = np.exp(-grid.kk * dt * 0.5)
+                    cosom = np.cosh(dt * np.abs(coupl))
+                    sinomm = np.sinh(dt * np.abs(coupl)) * np.exp(-1j * cang)
+                    sinomp = np.sinh(dt * np.abs(coupl)) * np.exp(1j * cang)
+                    iswitch = i
+            if i >= max_iter:                                    # 124-127 (cap 500000)
+                converged = False
+                break
+            i = i + 1
+        self.itp_iterations = i
+        self.itp_dt = dt
+        self.itp_converged = converged
+        out = [phi]
+        if nargout >= 2:                                         # 131-142: entries 1..i-1
+            out.append(np.array(MU[:i - 1]))
+        if nargout >= 3:
+            out.append(np.array(MU2[:i - 1]))
+        if nargout >= 4:
+            out.append(np.array(EE[:i - 1]))
+        self.init_state = phi                                    # 143-144
+        self.current_state = phi
+        return out[0] if nargout == 1 else tuple(out)
+
+
+class GPEtask2comp(GPEtaskSpinor):
+    """lib/@GPEtask2comp (GPEtask2comp.m:1-154): incoherent two-component mixture,
+    per-component particle numbers N1, N2 and masses M1, M2 (omega = 0).
+
+    Its solve_split.m is the same file as GPEtaskSpinor's, which reads
+    task.coupling and task.nlincpl -- properties GPEtask2comp does not define, so
+    the reference raises there.  Here both are fixed at 0 (the evident intent: no
+    inter-component coupling), so solve_split runs the uncoupled step."""
+
+    def __init__(self, grid, trappot, seed=0):
+        super().__init__(grid, trappot, seed)                    # GPEtask2comp.m:16-26
+        self.ncomp = 2
+        self.N1 = None
+        self.N2 = None
+        self.mu1 = None
+        self.mu2 = None
+        self.M1 = 1.0
+        self.M2 = 1.0
+
+    def applyham1c(self, phi, j, time=None):
+        # GPEtask2comp.m:51-64: no coupling term
+        self._check_omega()
+        if time is None:
+            time = self.current_time
+        res = self.getVtotal(time)[j]
+        g = np.asarray(self.g)
+        for k in range(self.ncomp):
+            res = res + g[j, k] * np.abs(phi[k]) ** 2
+        return res * phi[j] + self.grid.lap(phi[j])
+
+    def get_mu(self, phi=None, time=None):
+        # GPEtask2comp.m:76-86
+        if phi is None:
+            phi = self.current_state
+        if time is None:
+            time = self.current_time
+        return [np.real(self.grid.inner(phi[0], self.applyham1c(phi, 0, time)) / self.N1),
+                np.real(self.grid.inner(phi[1], self.applyham1c(phi, 1, time)) / self.N2)]
+
+    def groundstate_itp(self, dt, eps, phi=None, nargout=1, max_iter=500000):
+        """@GPEtask2comp/groundstate_itp.m:1-157 (omega = 0): K_j/2 . P . K_j/2 with
+        kinetic factors exp(-kk dt/2/M_j), densities taken before the first kinetic
+        half step, each component renormalised to N1 / N2; MU2 holds component 2's
+        log(mu2)/dt and is the convergence series unless nargout >= 4."""
+        self._check_omega()
+        grid = self.grid
+        VV = self.getVtotal(0)                                   # 22
+        g = np.asarray(self.g)
+        phi = self._itp_init(phi, 2)                             # 26-28
+        ekk1 = np.exp(-grid.kk * dt * 0.5 / self.M1)             # 29-30
+        ekk2 = np.exp(-grid.kk * dt * 0.5 / self.M2)
+        MU1, MU2, EE = [], [], []
+        i = 1
+        iswitch = 20
+        converged = True
+        while True:
+            t0 = VV[0] + g[0, 0] * np.abs(phi[0]) ** 2 + g[0, 1] * np.abs(phi[1]) ** 2   # 58
+            t1 = VV[1] + g[1, 0] * np.abs(phi[0]) ** 2 + g[1, 1] * np.abs(phi[1]) ** 2   # 59
+            phi[0] = grid.ifft(ekk1 * grid.fft(phi[0]))          # 67-68
+            phi[1] = grid.ifft(ekk2 * grid.fft(phi[1]))
+            phi[0] = np.exp(-t0 * dt) * phi[0]                   # 70-71
+            phi[1] = np.exp(-t1 * dt) * phi[1]
+            phi[0] = grid.ifft(ekk1 * grid.fft(phi[0]))          # 78-79
+            phi[1] = grid.ifft(ekk2 * grid.fft(phi[1]))
+            n1c = np.real(grid.integrate(phi[0] * np.conj(phi[0])))   # 82-85
+            n2c = np.real(grid.integrate(phi[1] * np.conj(phi[1])))
+            mu1 = np.sqrt(self.N1 / n1c)
+            mu2 = np.sqrt(self.N2 / n2c)
+            MU1.append(np.log(mu1) / dt)                         # 86-87
+            MU2.append(np.log(mu2) / dt)
+            phi[0] = phi[0] * mu1                                # 89-90
+            phi[1] = phi[1] * mu2
+            self.current_state = phi
+            if nargout >= 4:                                     # 98-102
+                EE.append(self.get_energy(phi) / self.Ntotal)
+            else:
+                EE.append(MU2[-1])
+            if (i - iswitch) > 10 and i % 10 == 0:               # 113-135
+                delta = (abs(EE[i - 1] - EE[i - 10]) / 9 + abs(EE[i - 1] - EE[i - 2])) / dt
+                if delta < eps:
+                    if dt < eps or dt < 5e-6:
+                        break
+                    dt = dt / 1.5
+                    ekk1 = np.exp(-grid.kk * dt * 0.5 / self.M1)
+                    ekk2 = np.exp(-grid.kk * dt * 0.5 / self.M2)
+                    iswitch = i
+            if i >= max_iter:                                    # 136-139
+                converged = False
+                break
+            i = i + 1
+        self.itp_iterations = i
+        self.itp_dt = dt
+        self.itp_converged = converged
+        out = [phi]
+        if nargout >= 2:                                         # 143-154
+            out.append(np.array(MU1[:i - 1]))
+        if nargout >= 3:
+            out.append(np.array(MU2[:i - 1]))
+        if nargout >= 4:
+            out.append(np.array(EE[:i - 1]))
+        self.init_state = phi                                    # 155-156
+        self.current_state = phi
+        return out[0] if nargout == 1 else tuple(out)

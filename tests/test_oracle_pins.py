@@ -9,6 +9,7 @@ sign, an index, a transposed operand, a wrong block phase) fails it.
 import numpy as np
 import pytest
 
+import oracle
 from oracle import Grid2D, Grid3D, GPEtask, GPEtaskSpinor
 from conftest import rel_l2
 
@@ -259,3 +260,67 @@ def test_spinor_norm_and_reduction_to_scalar():
     out = t.solve_split(0.005, 10, 2)
     n = grid.integrate(np.abs(out[0]) ** 2 + np.abs(out[1]) ** 2)
     assert abs(n / t.Ntotal - 1) < 1e-13
+
+
+def _two_comp_itp_task(cls, n=32, xmax=6.0, g=((0.0, 0.0), (0.0, 0.0))):
+    grid = oracle.grid2d(xmax, n, xmax, n)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    t = cls(grid, [V, V])
+    t.g = np.array(g)
+    return t, grid
+
+
+def test_spinor_itp_coupled_oscillator_ground_state():
+    # g = 0, uniform Omega: H = h_osc (x) 1 + [[0, Omega], [conj(Omega), 0]]; the ground
+    # state has mu = 1 - |Omega| and equal populations with psi2/psi1 = -exp(-i arg Omega)
+    t, grid = _two_comp_itp_task(oracle.GPEtaskSpinor)
+    om = 0.3 * np.exp(0.7j)
+    t.coupling = om
+    t.Ntotal = 2.0
+    phi, MU, MU2, EE = t.groundstate_itp(0.02, 1e-9, nargout=4)
+    assert t.itp_converged
+    assert abs(MU[-1] - (1 - abs(om))) < 1e-6
+    assert abs(MU2[-1] - (1 - abs(om))) < 1e-6 and abs(EE[-1] - (1 - abs(om))) < 1e-6
+    # the energy converges quadratically in the state error, so the state is held
+    # to the square root of the energy tolerance
+    n1, n2 = grid.integrate(np.abs(phi[0]) ** 2), grid.integrate(np.abs(phi[1]) ** 2)
+    assert abs(n1 - 1.0) < 2e-5 and abs(n2 - 1.0) < 2e-5 and abs(n1 + n2 - 2.0) < 1e-12
+    ratio = grid.inner(phi[0], phi[1]) / n1
+    assert abs(ratio - (-np.exp(-1j * np.angle(om)))) < 2e-5
+
+
+def test_2comp_itp_masses_and_separate_norms():
+    # g = 0: each component relaxes to its own oscillator ground state, energy 1/sqrt(M)
+    # in 2D; each is renormalised to its own particle number
+    t, grid = _two_comp_itp_task(oracle.GPEtask2comp)
+    t.N1, t.N2, t.M2 = 3.0, 0.5, 2.0
+    phi, MU1, MU2 = t.groundstate_itp(0.02, 1e-9, nargout=3)
+    assert t.itp_converged
+    assert abs(MU1[-1] - 1.0) < 1e-6
+    assert abs(MU2[-1] - 1.0 / np.sqrt(2.0)) < 1e-6
+    assert abs(grid.integrate(np.abs(phi[0]) ** 2) - 3.0) < 1e-12
+    assert abs(grid.integrate(np.abs(phi[1]) ** 2) - 0.5) < 1e-12
+
+
+def test_two_comp_itp_history_lengths_and_cap():
+    # the histories hold entries 1..i-1 of the i iterations run (groundstate_itp.m:131-142)
+    t, _ = _two_comp_itp_task(oracle.GPEtaskSpinor, n=16)
+    t.coupling, t.Ntotal = 0.2, 1.0
+    _, MU = t.groundstate_itp(0.01, 1e-30, nargout=2, max_iter=37)
+    assert not t.itp_converged and t.itp_iterations == 37 and len(MU) == 36
+
+
+def test_2comp_solve_split_is_uncoupled_spinor_step():
+    # the reference GPEtask2comp.solve_split is GPEtaskSpinor's with coupling 0
+    grid = oracle.grid2d(5, 16, 5, 16)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    rng = np.random.Generator(np.random.PCG64(8))
+    psi = [np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2) * np.exp(1j * rng.uniform(0, 1, grid.size))
+           for _ in range(2)]
+    a, b = oracle.GPEtask2comp(grid, [V, V]), oracle.GPEtaskSpinor(grid, [V, V])
+    for t in (a, b):
+        t.g = np.array([[0.3, 0.2], [0.2, 0.4]])
+        t.Ntotal = 2.0
+        t.init_state = [p.copy() for p in psi]
+    pa, pb = a.solve_split(0.01, 10, 2), b.solve_split(0.01, 10, 2)
+    assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
