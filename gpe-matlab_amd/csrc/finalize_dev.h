@@ -49,6 +49,34 @@ __device__ __forceinline__ void finalize_core(const FinParams& f, double* red) {
   const double sA = tot[0], sB = tot[1], sC = tot[2], kx = tot[3], ky = tot[4], kz = tot[5];
   const double c12 = tot[6], c2 = tot[7], sx = tot[8], kx2 = tot[9], ky2 = tot[10], kz2 = tot[11];
   if (threadIdx.x != 0) return;
+  if (f.mode == FIN_ITP2) {
+    // GPEtaskSpinor/groundstate_itp.m:82-88, GPEtask2comp/groundstate_itp.m:82-90
+    if (f.mixture) {
+      const double mu1 = sqrt(f.n1t / (sA * f.w)), mu2 = sqrt(f.n2t / (sB * f.w));
+      st->sc[0] = mu1;
+      st->sc[1] = mu2;
+      f.MU[f.iter] = log(mu1) / f.dt;
+      f.MU2[f.iter] = log(mu2) / f.dt;
+    } else {
+      const double mu = sqrt(f.n1t / ((sA + sB) * f.w));
+      st->sc[0] = mu;
+      st->sc[1] = mu;
+      f.MU[f.iter] = log(mu) / f.dt;
+    }
+    return;
+  }
+  if (f.mode == FIN_ITP2M) {
+    // real(inner(phi, applyham(phi))) and get_energy (g/2) over Ntotal
+    // (GPEtaskSpinor/groundstate_itp.m:91-100); x sums come from the scaled state,
+    // the y/z kinetic sums from the unscaled one
+    const double q1 = st->sc[0] * st->sc[0], q2 = st->sc[1] * st->sc[1];
+    const double sKs = (kx + q1 * (ky + kz)) + (kx2 + q2 * (ky2 + kz2));
+    const double sCf = (f.g11 * sC + (f.g12 + f.g21) * c12) + f.g22 * c2;
+    const double base = (sB + sKs) + 2.0 * sx;
+    if (f.which & 1) f.MU2[f.iter] = (base + sCf) * f.w / f.Ntotal;
+    if (f.which & 2) f.EE[f.iter] = (base + 0.5 * sCf) * f.w / f.Ntotal;
+    return;
+  }
   const double sK = ((kx + ky) + kz) + ((kx2 + ky2) + kz2);
   st->sA = sA; st->sB = sB; st->sK = sK;
   // scalar runs: sC = sum |psi|^4; spinor runs: sC = sum_jk g_jk n_j n_k, sX = sum Re conj(psi1) Omega psi2

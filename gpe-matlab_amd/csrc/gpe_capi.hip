@@ -155,6 +155,8 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->adj_grad);
   (void)hipFree(h->kcol_c);
   (void)hipFree(h->kax_c);
+  (void)hipFree(h->itp2_P);
+  for (double* b : h->itp2_hist) (void)hipFree(b);
   for (int i = 0; i < 2; ++i) (void)hipFree(h->rb_lam[i]);
   (void)hipFree(h->rb_ckpt);
   (void)hipFree(h->rb_tape);

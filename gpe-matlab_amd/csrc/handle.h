@@ -81,6 +81,10 @@ struct gpe_handle {
   int adj_gradcap = 0;
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
+  // two-component ITP (gpe_itp2)
+  double* itp2_P = nullptr;       // stored phase factors, 2 x npts
+  double* itp2_hist[3] = {nullptr, nullptr, nullptr};   // MU, MU2, EE device histories
+  int itp2_cap = 0;
   // spinor Rabi adjoint (gpe_rabi_forward, gpe_rabi_grad)
   double2* rb_lam[2] = {nullptr, nullptr};
   double2* rb_ckpt = nullptr;     // checkpoints, 2 components each

@@ -2,6 +2,7 @@
 #pragma once
 #include "sweep.h"
 struct gpe_handle;
+namespace gpe { struct FinParams; }
 
 namespace gpe_spinor {
 gpe::SpinorParams spinor_params(gpe_handle* h);
@@ -14,6 +15,8 @@ int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax
 // components before each step (2 * n_rec arrays); reduce = false skips the
 // block-end chemical potential and renormalisation.
 int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce);
+// SP_KEX + y/z kinetic sums of both components into the partials; fills f's counts.
+int spinor_reduce(gpe_handle* h, gpe::FinParams* f, bool scaled);
 }  // namespace gpe_spinor
 
 namespace gpe {

@@ -47,6 +47,7 @@ gpe::SpinorParams spinor_params(gpe_handle* h) {
   p.sinomp = make_double2(h->sinomp[0], h->sinomp[1]);
   p.nlincpl = h->prm.nlincpl;
   p.keax = a.keax;
+  p.npts = (unsigned)h->npts;
   return p;
 }
 
@@ -83,10 +84,13 @@ void rabi_scalars(gpe_handle* h) {
   h->sinomp[0] = pl.real(); h->sinomp[1] = pl.imag();
 }
 
-// Reductions for ncur and mu on the real-space state of both components.
-int spinor_reduce(gpe_handle* h, FinParams* f) {
+// Reductions for ncur and mu on the real-space state of both components
+// (scaled: each component multiplied by st->sc[k] in the x-sweep).
+int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled) {
   int rc, nb = 0;
-  if ((rc = spinor_sweep(h, gpe::SP_KEX, spinor_params(h), &nb))) return rc;
+  gpe::SpinorParams p = spinor_params(h);
+  p.apply_sc = scaled ? 1 : 0;
+  if ((rc = spinor_sweep(h, gpe::SP_KEX, p, &nb))) return rc;
   for (int s : {gpe::SL_A, gpe::SL_B, gpe::SL_C, gpe::SL_C12, gpe::SL_C2, gpe::SL_X, gpe::SL_KX, gpe::SL_KX2})
     f->cnt[s] = nb;
   for (int comp = 0; comp < 2; ++comp)
@@ -148,7 +152,7 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
     if ((rc = sweep(h, 0, gpe::S_INV, base_params(h, 0, h->psi[comp], nullptr)))) return rc;
   if (!reduce) return GPE_OK;
   FinParams f = fin_params(h, gpe::FIN_SPINOR);
-  if ((rc = spinor_reduce(h, &f))) return rc;
+  if ((rc = spinor_reduce(h, &f, false))) return rc;
   f.renorm = h->prm.gamma > 0 ? 1 : 0;
   f.NN0 = h->NN0;
   f.tau = h->prm.decay_rate;
@@ -166,7 +170,7 @@ namespace gpe {
 int spinor_measure(gpe_handle* h, double out[5]) {
   int rc;
   FinParams f = fin_params(h, FIN_SUMS);
-  if ((rc = spinor_reduce(h, &f))) return rc;
+  if ((rc = spinor_reduce(h, &f, false))) return rc;
   HIPCHK(h, launch_finalize(f, h->stream));
   DevState s;
   if ((rc = read_state(h, &s))) return rc;

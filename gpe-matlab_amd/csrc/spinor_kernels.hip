@@ -12,6 +12,13 @@
 // SP_KEX: reductions of the block-end chemical potential on the real-space
 //         state (solve_split.m:100-115, GPEtaskSpinor.m:55-92) plus the x part of
 //         the kinetic energy of both components.
+// SP_ITPA/B/C: the x-sweeps of one two-component imaginary-time iteration
+//         (GPEtaskSpinor/groundstate_itp.m:49-88, GPEtask2comp/groundstate_itp.m:58-90):
+//         A renormalises the real-space state, stores exp(-(V_k + sum_j g_kj n_j) dt)
+//         from that density and transforms forward; B (after a kinetic half step)
+//         transforms back, applies the stored factor (and the Rabi rotation) and
+//         transforms forward; C (after the second half step) transforms back and
+//         reduces each component's norm.
 // SP_ADJ: adjoint of SP_PR for gamma = 0 and Omega(x) = om0 exp(i kw x): inverse
 //         x-FFT of both adjoint components, the transposed step linearised at the
 //         stored pre-step state, forward x-FFT, and the per-work-group sums of
@@ -81,9 +88,10 @@ __device__ __forceinline__ double2 xchg(double2 v) { return make_double2(xchg(v.
 // Coupling kinds of SP_PR, fixed per launch.
 // CK_ARRAY_REAL: array Omega with dt = i di; CK_ARRAY_TAYLOR: the same with
 // di^2 max|Omega|^2 <= 0.6 (rabi_maxz), where the Rabi factors are power series.
-// CK_SCALAR_REAL: scalar Omega with dt = i di.
+// CK_SCALAR_REAL: scalar Omega with dt = i di.  CK_NONE: no coupling (mixture).
 enum CplKind : int {
-  CK_SCALAR = 0, CK_ARRAY = 1, CK_NONLIN = 2, CK_ARRAY_REAL = 3, CK_ARRAY_TAYLOR = 4, CK_SCALAR_REAL = 5
+  CK_SCALAR = 0, CK_ARRAY = 1, CK_NONLIN = 2, CK_ARRAY_REAL = 3, CK_ARRAY_TAYLOR = 4, CK_SCALAR_REAL = 5,
+  CK_NONE = 6
 };
 
 #ifndef GPE_ADJ_WAVES
@@ -253,9 +261,80 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       p.gacc[blockIdx.x] += r0;
       p.gacc[p.maxwg + blockIdx.x] += r1;
     }
+  } else if constexpr (SEQ == SP_ITPA) {
+    const double sk = p.st->sc[k];
+    const double gk1 = k ? p.g21 : p.g11, gk2 = k ? p.g22 : p.g12;
+    double* __restrict__ pk = p.pbuf + (k ? p.npts : 0u);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const unsigned idx = base + (unsigned)m * step;
+      v[m] = cscale(v[m], sk);                                   // phi{j} .* mu (88)
+      const double nown = cabs2(v[m]);
+      const double noth = xchg(nown);
+      const double n1 = k ? noth : nown, n2 = k ? nown : noth;
+      const double a = (Vk[idx] + gk1 * n1) + gk2 * n2;         // 49-54
+      if (active) pk[idx] = exp_cw(-a * p.dr);                 // exp(-tmp2 dt) (70-71)
+    }
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+    }
+  } else if constexpr (SEQ == SP_ITPB) {
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    const double* __restrict__ pk = p.pbuf + (k ? p.npts : 0u);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const unsigned idx = base + (unsigned)m * step;
+      const double f = pk[idx];
+      if constexpr (CK == CK_NONE) {
+        v[m] = cscale(v[m], f);                                  // GPEtask2comp 70-71
+      } else {
+        const double2 o = xchg(v[m]);
+        double2 cosom, soff;
+        if constexpr (CK == CK_ARRAY) {                          // cosh/sinh(dt |Omega|), dt real
+          const double2 om = p.cpl[idx];
+          const double r = hypot(om.x, om.y);
+          double2 sh;
+          ccoshsinh(p.dr * r, 0.0, &cosom, &sh);
+          const double2 ph = r > 0.0 ? make_double2(om.x / r, om.y / r) : make_double2(1.0, 0.0);
+          soff = k ? cmulc(sh, ph) : cmul(sh, ph);
+        } else {
+          cosom = p.cosom;
+          soff = k ? p.sinomm : p.sinomp;
+        }
+        v[m] = cscale(csub(cmul(cosom, v[m]), cmul(soff, o)), f);   // GPEtaskSpinor 70-71
+      }
+    }
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+    }
+  } else if constexpr (SEQ == SP_ITPC) {
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) acc += cabs2(v[m]);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+    }
+    if (!active) acc = 0.0;
+    const double r1 = block_sum(k ? 0.0 : acc, red);
+    const double r2 = block_sum(k ? acc : 0.0, red);
+    if (tid == 0) {
+      p.partials[SL_A * p.maxwg + blockIdx.x] = r1;
+      p.partials[SL_B * p.maxwg + blockIdx.x] = r2;
+    }
   } else {
     double accA = 0.0, accB = 0.0, accC1 = 0.0, accC12 = 0.0, accC2 = 0.0, accX = 0.0, k1 = 0.0, k2 = 0.0;
     const double own1 = k ? 0.0 : 1.0;   // sums that belong to component 1's lane only
+    if (p.apply_sc) {
+      const double sk = p.st->sc[k];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cscale(v[m], sk);
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const unsigned idx = base + (unsigned)m * step;
@@ -330,6 +409,17 @@ hipError_t GPE_CAT(launch_spinor_sweep_, GPE_N)(int seq, const SpinorParams& p, 
     hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEX, CK_SCALAR>), g, b, 0, s, p);
   } else if (seq == SP_ADJ) {
     hipLaunchKernelGGL((spinor_adj_kernel<N, C2>), g, b, 0, s, p);
+  } else if (seq == SP_ITPA) {
+    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ITPA, CK_NONE>), g, b, 0, s, p);
+  } else if (seq == SP_ITPB) {
+    if (p.mixture)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ITPB, CK_NONE>), g, b, 0, s, p);
+    else if (p.cpl)
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ITPB, CK_ARRAY>), g, b, 0, s, p);
+    else
+      hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ITPB, CK_SCALAR>), g, b, 0, s, p);
+  } else if (seq == SP_ITPC) {
+    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_ITPC, CK_NONE>), g, b, 0, s, p);
   } else {
     return hipErrorInvalidValue;
   }

@@ -4,7 +4,11 @@
 
 namespace gpe {
 
-enum FinMode : int { FIN_SUMS = 0, FIN_BLOCK = 1, FIN_ITP = 2, FIN_MAX = 3, FIN_SPINOR = 4 };
+enum FinMode : int {
+  FIN_SUMS = 0, FIN_BLOCK = 1, FIN_ITP = 2, FIN_MAX = 3, FIN_SPINOR = 4,
+  FIN_ITP2 = 5,    // two-component ITP renormalisation and MU histories
+  FIN_ITP2M = 6    // two-component ITP applyham / energy histories
+};
 
 struct FinParams {
   const double* partials;
@@ -21,6 +25,12 @@ struct FinParams {
   double* MU;          // FIN_ITP: device history
   int iter;
   double g11, g12, g21, g22;   // FIN_SPINOR: interaction matrix
+  // FIN_ITP2 / FIN_ITP2M
+  int mixture;         // per-component norms N1, N2 (else one total norm n1t)
+  double n1t, n2t;     // target norms
+  double* MU2;         // device histories, entry `iter`
+  double* EE;
+  int which;           // FIN_ITP2M: 1 = write MU2, 2 = write EE
 };
 
 hipError_t launch_finalize(const FinParams& f, hipStream_t s);

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(256) itp_check_kernel(const double2* __restric
 // pending scale when it materialises it.
 __global__ void __launch_bounds__(256) scale_kernel(double2* __restrict__ psi, DevState* st, int which,
                                                     long long n) {
-  const double s = which == 0 ? st->scale : st->mu_itp;
+  const double s = which == 0 ? st->scale : which == 1 ? st->mu_itp : st->sc[which - 2];
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     psi[i] = cscale(psi[i], s);
 }

@@ -45,6 +45,7 @@ struct DevState {
   double delta;   // ITP max-norm change
   double mu_itp;  // ITP renormalisation factor of the last iteration
   double pad[4];
+  double sc[2];   // two-component ITP: pending renormalisation factor of each component
 };
 
 struct SweepParams {
@@ -69,7 +70,7 @@ struct SweepParams {
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).
-enum SpSeq : int { SP_PR = 0, SP_KEX = 1, SP_ADJ = 2 };
+enum SpSeq : int { SP_PR = 0, SP_KEX = 1, SP_ADJ = 2, SP_ITPA = 3, SP_ITPB = 4, SP_ITPC = 5 };
 
 struct SpinorParams {
   double2* psi1;
@@ -95,6 +96,11 @@ struct SpinorParams {
   const double* xc;          // SP_ADJ: x coordinate per x index
   double om0, kw;            // SP_ADJ: coupling amplitude and wave number
   double* gacc;              // SP_ADJ: per-work-group accumulators of dL/d om0 [0, maxwg) and dL/d kw
+  // two-component imaginary time (dt = dr real):
+  double* pbuf;              // SP_ITPA writes, SP_ITPB reads exp(-(V_k + g_k1 n1 + g_k2 n2) dt), 2 x npts
+  unsigned npts;             // grid points (offset of component 2 in pbuf)
+  int apply_sc;              // SP_ITPA, SP_KEX: multiply the loaded state by st->sc[k] first
+  int mixture;               // SP_ITPB: no Rabi rotation (incoherent mixture)
 };
 
 // Launch one sweep; returns hipErrorInvalidValue for an unsupported length.

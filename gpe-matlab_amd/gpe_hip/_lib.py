@@ -38,6 +38,12 @@ class RabiArgs(C.Structure):
                 ("k", C.c_double), ("tape_gib", C.c_double)]
 
 
+class Itp2Args(C.Structure):
+    _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("nargout", C.c_int),
+                ("mixture", C.c_int), ("N1", C.c_double), ("N2", C.c_double), ("M1", C.c_double),
+                ("M2", C.c_double)]
+
+
 class ItpArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("min_iter", C.c_int),
                 ("adjust_stepsize", C.c_int), ("nargout", C.c_int)]
@@ -61,6 +67,7 @@ SIGNATURES = {
     "gpe_split_block": [_P, _D, _DP, _DP],
     "gpe_measure": [_P, _DP],
     "gpe_itp": [_P, C.POINTER(ItpArgs), _DP, _DP, _DP, _IP, _DP, _IP],
+    "gpe_itp2": [_P, C.POINTER(Itp2Args), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_device_info": [C.c_char_p, C.c_int],
     "gpe_profile_enable": [_P, C.c_int],
     "gpe_profile_read": [_P, _DP, _IP, C.c_int],
@@ -236,6 +243,19 @@ class Device:
         self.call("gpe_itp", C.byref(a), dptr(MU), dptr(MU2), dptr(EE), C.byref(nit), C.byref(dtf), C.byref(conv))
         i = nit.value
         return MU[:i], MU2[:i], EE[:i], i, dtf.value, bool(conv.value)
+
+    def itp2(self, dt, eps, max_iter, nargout, mixture=False, N1=0.0, N2=0.0, M1=1.0, M2=1.0):
+        """Two-component ITP from the uploaded state: (MU, MU2, EE, iterations, final dt, converged)."""
+        a = Itp2Args(float(dt), float(eps), int(max_iter), int(nargout), int(bool(mixture)), float(N1), float(N2),
+                     float(M1), float(M2))
+        MU = np.zeros(max_iter)
+        MU2 = np.zeros(max_iter)
+        EE = np.zeros(max_iter)
+        nit, conv = C.c_int(), C.c_int()
+        dtf = C.c_double()
+        self.call("gpe_itp2", C.byref(a), dptr(MU), dptr(MU2), dptr(EE), C.byref(nit), C.byref(dtf), C.byref(conv))
+        i = nit.value
+        return MU[:i - 1], MU2[:i - 1], EE[:i - 1], i, dtf.value, bool(conv.value)
 
     def control_set(self, V0, basis):
         v0 = to_colmajor_f64(np.broadcast_to(V0, self.shape))

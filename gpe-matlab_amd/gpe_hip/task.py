@@ -392,3 +392,87 @@ class GPEtaskSpinor(GPEtask):
                     mu, ncur = dev.split_block(time2)
             self.ext_callback(j, time2, mu, ncur)
         return self.current_state
+
+    # two-component imaginary time ------------------------------------------------
+    itp_max_iter_2c = 500000   # the two-component solvers stop at 500000 iterations
+
+    def _itp_init(self, phi, ncomp):
+        # GPEtaskSpinor/groundstate_itp.m:26-28 (rand, from the task's seeded generator)
+        if phi is None:
+            g = self.grid
+            phi = []
+            for _ in range(ncomp):
+                r = self.rng.random(g.size) + 1j * self.rng.random(g.size)
+                phi.append(g.normalize(r) / np.sqrt(ncomp))
+        return [np.asarray(p, dtype=np.complex128) for p in phi]
+
+    def _itp2_finish(self, res, nargout, names):
+        MU, MU2, EE, i, dt_end, conv = res
+        if not conv:
+            warnings.warn("Convergence not reached")
+        self._dev_state_valid = True
+        self._host_stale = True
+        phi = self.current_state
+        self.itp_iterations = i
+        self.itp_dt = dt_end
+        out = [phi]
+        series = {"MU": MU, "MU2": MU2, "EE": EE}
+        for n in names[:max(0, nargout - 1)]:
+            out.append(series[n])
+        self.init_state = phi
+        return out[0] if nargout == 1 else tuple(out)
+
+    def groundstate_itp(self, dt, eps, phi0=None, nargout=1):
+        """@GPEtaskSpinor/groundstate_itp.m:1-145 on the GPU (omega = 0).  Returns phi
+        and, by ``nargout``, MU, MU2 (applyham-based) and EE (energy) over Ntotal."""
+        self._check_omega()
+        dev = self._device()
+        ncomp = np.asarray(self.g).shape[0]                      # 20-21
+        self.ncomp = ncomp
+        VV = self.getVtotal(0)                                   # 22
+        phi = self._itp_init(phi0, ncomp)                        # 26-28
+        self._set_params(self.Ntotal)
+        self._upload_potential(VV)
+        self._upload_coupling()
+        self._upload(phi)
+        res = dev.itp2(dt, eps, self.itp_max_iter_2c, nargout)
+        return self._itp2_finish(res, nargout, ["MU", "MU2", "EE"])
+
+
+class GPEtask2comp(GPEtaskSpinor):
+    """Drop-in for lib/@GPEtask2comp (GPEtask2comp.m:1-154): incoherent two-component
+    mixture with particle numbers N1, N2 and masses M1, M2.
+
+    Deviation: its solve_split.m is the same file as GPEtaskSpinor's and reads
+    task.coupling / task.nlincpl, properties this class does not define, so the
+    reference raises there; here both stay 0 and solve_split runs the uncoupled
+    two-component step."""
+
+    def __init__(self, grid, trappot, seed=None):
+        super().__init__(grid, trappot, seed)                    # GPEtask2comp.m:16-26
+        self.ncomp = 2
+        self.N1 = None
+        self.N2 = None
+        self.mu1 = None
+        self.mu2 = None
+        self.M1 = 1.0
+        self.M2 = 1.0
+
+    def groundstate_itp(self, dt, eps, phi0=None, nargout=1):
+        """@GPEtask2comp/groundstate_itp.m:1-157 on the GPU (omega = 0).  Returns phi
+        and, by ``nargout``, MU1, MU2 (log(mu_j)/dt of each component) and EE
+        (energy over Ntotal)."""
+        self._check_omega()
+        if self.N1 is None or self.N2 is None:
+            raise ValueError("GPEtask2comp.groundstate_itp needs N1 and N2")
+        dev = self._device()
+        VV = self.getVtotal(0)                                   # 22
+        phi = self._itp_init(phi0, 2)                            # 26-28
+        self.coupling = 0.0
+        self._set_params(self.Ntotal)
+        self._upload_potential(VV)
+        self._upload_coupling()
+        self._upload(phi)
+        res = dev.itp2(dt, eps, self.itp_max_iter_2c, nargout, mixture=True, N1=self.N1, N2=self.N2,
+                       M1=self.M1, M2=self.M2)
+        return self._itp2_finish(res, nargout, ["MU", "MU2", "EE"])

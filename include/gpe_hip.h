@@ -113,6 +113,28 @@ typedef struct {
 int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, double* EE, int* n_iter,
             double* dt_final, int* converged);
 
+/* Two-component ground states after the initial state is uploaded (both
+ * components): GPEtaskSpinor.groundstate_itp (lib/@GPEtaskSpinor/groundstate_itp.m:1-145,
+ * mixture = 0: Rabi coupling from gpe_set_coupling, total norm Ntotal) and
+ * GPEtask2comp.groundstate_itp (lib/@GPEtask2comp/groundstate_itp.m:1-157,
+ * mixture = 1: no coupling, norms N1 / N2, masses M1 / M2).  omega = 0.  Caller
+ * arrays of max_iter entries receive the histories, entries 1..n_iter-1 as the
+ * reference returns them: MU (spinor MU, mixture MU1), MU2 (spinor: applyham-based,
+ * filled when nargout >= 3; mixture: log(mu2)/dt), EE (nargout >= 4; may be NULL
+ * otherwise).  converged = 0 when max_iter stopped the loop (the reference caps at
+ * 500000 and warns). */
+typedef struct {
+  double dt;
+  double eps;
+  int max_iter;
+  int nargout;     /* 1..4, as MATLAB's nargout */
+  int mixture;
+  double N1, N2;   /* mixture: particle numbers (GPEtask2comp.m:6-7) */
+  double M1, M2;   /* mixture: masses (GPEtask2comp.m:10-11) */
+} gpe_itp2_args;
+int gpe_itp2(gpe_handle* h, const gpe_itp2_args* a, double* MU, double* MU2, double* EE, int* n_iter,
+             double* dt_final, int* converged);
+
 /* ------------------------------------------------------------------ adjoint
  * (no reference counterpart: north-star form 3, the gradient entry point that
  * plays the torch.autograd.Function.backward role).
