@@ -20,6 +20,7 @@ Legs:
 """
 import argparse
 import json
+import warnings
 import os
 import sys
 import time
@@ -74,7 +75,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2"],
                     help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
                          "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
@@ -101,7 +102,7 @@ def main():
               "the multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
     if args.workload != "c2":
-        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "itp2": bench_itp2}[args.workload](args)
         return
 
     import gpe_hip
@@ -516,6 +517,63 @@ def bench_c5_adjoint(t, log, omega0=0.5, k=2.0, nsteps=1000, ddt=0.002, h=1e-6):
                 adjoint_forward_s=round(ef, 4), adjoint_grad_s=round(eg, 4),
                 adjoint_over_forward=round(eg / ef, 3), adjoint_same_loss=bool(L == Lg),
                 adjoint_sweeps_ms=split)
+
+
+def bench_itp2(args):
+    """Two-component imaginary time (GPEtaskSpinor.groundstate_itp) on the c5 grid:
+    192^3, V = 0.5(X^2+Y^2+Z^2) for both components, g = [0.02 0.018; 0.018 0.02],
+    Omega = 0.5 exp(2iX), Ntotal = 1e6, dt = 0.002, a fixed number of iterations."""
+    import gpe_hip
+    from gpe_hip import workloads
+    n = 192
+    G = gpe_hip.grid3d(12, n, 12, n, 12, n)
+    V = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2 + G.mesh.z ** 2)
+    tf, _ = workloads.thomas_fermi(G, V, 1e6, 0.02)
+    rng = np.random.Generator(np.random.PCG64(7))
+    a = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    b = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    gm = np.array([[0.02, 0.018], [0.018, 0.02]])
+    om = 0.5 * np.exp(2j * G.mesh.x)
+    t = gpe_hip.GPEtaskSpinor(G, [V, V], seed=7)
+    t.g, t.coupling, t.Ntotal = gm, om, 1e6
+    parity = None
+    if not args.no_parity:
+        import oracle
+        O = oracle.grid3d(12, n, 12, n, 12, n)
+        to = oracle.GPEtaskSpinor(O, [V, V], seed=7)
+        to.g, to.coupling, to.Ntotal = gm, om, 1e6
+        t.itp_max_iter_2c = 3
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            pg = t.groundstate_itp(0.002, 1e-30, [a, b])
+        c0 = time.perf_counter()
+        po = to.groundstate_itp(0.002, 1e-30, [a, b], max_iter=3)
+        cel = time.perf_counter() - c0
+        parity = max(float(np.linalg.norm(pg[i] - po[i]) / np.linalg.norm(po[i])) for i in range(2))
+        log(f"itp2 parity (3 iterations) rel-L2 {parity:.2e}; oracle {cel:.1f}s")
+    iters = max(20, args.steps)
+    dev = t._device()
+    t._set_params(1e6)
+    t._upload_potential([V, V])
+    t._upload_coupling()
+    t._upload([a, b])
+    dev.itp2(0.002, 1e-30, 10, 1)                          # warm-up
+    t._upload([a, b])
+    sync()
+    t0 = time.perf_counter()
+    dev.itp2(0.002, 1e-30, iters, 1)                       # device loop only, state resident
+    sync()
+    el = time.perf_counter() - t0
+    rate = iters / el
+    bpt = 2 * (48 + 96 + 40 + 96 + 32)   # per component: A, K/2, B, K/2, C
+    gbps = bpt * n ** 3 * rate / 1e9
+    log(f"itp2: {iters} iterations in {el:.3f}s -> {rate:.1f} it/s, {gbps:.0f} GB/s algorithmic")
+    _line("two-component ITP iterations/s (GPEtaskSpinor.groundstate_itp 3D 192^3)", round(rate, 2), "iterations/s",
+          iters, 10, el, True, {"workload": "GPEtaskSpinor.groundstate_itp grid3d(12,192,...) V=0.5(X^2+Y^2+Z^2), "
+                                            "g=[0.02 0.018;0.018 0.02], Omega=0.5 exp(2iX), Ntotal=1e6, dt=0.002, "
+                                            "nargout=1, fixed iteration count", "grid": [n] * 3},
+          iter_alg_GBps=round(gbps, 1), iter_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
+          alg_bytes_per_point=bpt, parity_rel_l2_vs_oracle_3_iterations=parity)
 
 
 if __name__ == "__main__":
