@@ -1,4 +1,4 @@
-"""Oracle solvers: restatement of lib/@GPEtask and lib/@GPEtaskSpinor (omega = 0 paths).
+"""Oracle solvers: restatement of lib/@GPEtask, lib/@GPEtaskSpinor and lib/@GPEtask2comp.
 
 Test infrastructure only (see ``oracle/__init__.py``).  The control flow, the order
 of the pointwise operations and the block structure follow the MATLAB files line by
@@ -8,7 +8,9 @@ Deviations, all forced by the environment and stated here:
   * ``rng('shuffle')`` (GPEtask.m:49) is replaced by a seeded numpy PCG64 generator
     (``task.seed``), so random initial guesses are reproducible.
   * ``dispstat``/``toc``/``imagesc`` console and plot output is not reproduced.
-  * Only ``omega == 0`` (no rotating frame) is restated; other values raise.
+  * The rotating frame is restated for a scalar ``omega`` on GPEtask (set_kinop.m:5-7,
+    ssft_kin_step.m:22-25, lap with omega); an array ``omega`` (the Crank-Nicolson
+    Lz branch) and omega != 0 on the two-component classes raise.
 """
 import numpy as np
 
@@ -58,6 +60,10 @@ class GPEtask:
         if not np.isscalar(self.omega) or self.omega != 0:
             raise NotImplementedError("rotating frame (omega != 0) is outside the restated path")
 
+    def _check_scalar_omega(self):
+        if not np.isscalar(self.omega):
+            raise NotImplementedError("an array omega (Crank-Nicolson Lz step) is outside the restated path")
+
     def getVtotal(self, time):
         # GPEtask.m:62-71
         if time == 0 and np.size(self.V0) > 1:
@@ -73,14 +79,17 @@ class GPEtask:
         return self.applyh0(phi, time) + self.g * np.abs(phi) ** 2 * phi
 
     def applyh0(self, phi=None, time=None):
-        # GPEtask.m:80-95 (omega == 0 branch: V.*phi + lap(phi))
-        self._check_omega()
+        # GPEtask.m:80-95 (scalar omega: V.*phi + lap(phi, omega))
+        self._check_scalar_omega()
         if phi is None:
             phi = self.current_state
         if time is None:
             time = self.current_time
         res = self.getVtotal(time) * phi
-        res = res + self.grid.lap(phi)
+        if self.omega != 0:
+            res = res + self.grid.lap(phi, self.omega)
+        else:
+            res = res + self.grid.lap(phi)
         return res
 
     def get_energy(self, phi=None, time=None):
@@ -98,14 +107,24 @@ class GPEtask:
         return res
 
     def set_kinop(self, dt):
-        # set_kinop.m:8-10 (omega == 0): kinop = exp(-kk*dt)
-        self._check_omega()
-        self.kinop = np.exp(-self.grid.kk * dt)
+        # set_kinop.m:5-10
+        self._check_scalar_omega()
+        g = self.grid
+        if self.omega != 0:
+            self.kinop = np.exp(-(g.kx ** 2 / 2 - g.kx * g.mesh.y * self.omega) * dt / 2)      # 6
+            self.kinop2 = np.exp(-(g.ky ** 2 / 2 + g.kz ** 2 / 2 + g.ky * g.mesh.x * self.omega) * dt)   # 7
+        else:
+            self.kinop = np.exp(-g.kk * dt)                                                  # 9
 
     def ssft_kin_step(self, phi, dt):
-        # ssft_kin_step.m:17: phi = ifft(kinop .* fft(phi))
-        self._check_omega()
-        return self.grid.ifft(self.kinop * self.grid.fft(phi))
+        # ssft_kin_step.m:22-27
+        self._check_scalar_omega()
+        g = self.grid
+        if self.omega != 0:
+            phi = g.ifftx(self.kinop * g.fftx(phi))                                          # 23
+            phi = g.iffty(g.ifftz(self.kinop2 * g.fftz(g.ffty(phi))))                        # 24
+            return g.ifftx(self.kinop * g.fftx(phi))                                         # 25
+        return g.ifft(self.kinop * g.fft(phi))                                               # 27
 
     @staticmethod
     def _hist_set(arr, step, val):

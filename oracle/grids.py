@@ -1,4 +1,4 @@
-"""Oracle grids: restatement of lib/@grid1d, lib/@grid2d, lib/@grid3d (omega = 0 paths).
+"""Oracle grids: restatement of lib/@grid1d, lib/@grid2d, lib/@grid3d.
 
 Test infrastructure only (see ``oracle/__init__.py``).
 """
@@ -51,9 +51,31 @@ class _GridBase:
         # grid2d/ifft.m:11, grid3d/ifft.m:11: inverse with 1/N
         return sfft.ifftn(ifun, workers=WORKERS)
 
-    def lap(self, phi):
-        # grid2d/lap.m:6, grid3d/lap.m:6 (omega = 0):  ifft(kk .* fft(phi))
-        return self.ifft(self.kk * self.fft(phi))
+    # one-axis transforms: grid2d/fftx.m:11 (fft along dim 2 = x), ffty.m:11 (dim 1 = y),
+    # grid3d/fftz.m (dim 3); the inverses carry 1/n of their axis
+    def fftx(self, fun):
+        return sfft.fft(fun, axis=1, workers=WORKERS)
+
+    def ifftx(self, ifun):
+        return sfft.ifft(ifun, axis=1, workers=WORKERS)
+
+    def ffty(self, fun):
+        return sfft.fft(fun, axis=0, workers=WORKERS)
+
+    def iffty(self, ifun):
+        return sfft.ifft(ifun, axis=0, workers=WORKERS)
+
+    def lap(self, phi, omega=0.0):
+        # grid2d/lap.m:5-10, grid3d/lap.m:5-11.  omega = 0: ifft(kk .* fft(phi)); else the
+        # rotating-frame kinetic operator, one axis at a time:
+        #   ifftx((kx^2/2 - kx y omega) fftx) + iffty((ky^2/2 + ky x omega) ffty) [+ ifftz(kz^2/2 fftz)]
+        if omega == 0:
+            return self.ifft(self.kk * self.fft(phi))
+        res = self.ifftx((self.kx ** 2 / 2 - self.kx * self.mesh.y * omega) * self.fftx(phi)) + \
+            self.iffty((self.ky ** 2 / 2 + self.ky * self.mesh.x * omega) * self.ffty(phi))
+        if self.ndims == 3:
+            res = res + self.ifftz((self.kz ** 2 / 2) * self.fftz(phi))
+        return res
 
 
 class Grid2D(_GridBase):
@@ -79,6 +101,13 @@ class Grid2D(_GridBase):
         self.mesh = SimpleNamespace(x=X, y=Y, z=0, x2=X, y2=Y)
         self.weight = self.dx * self.dy           # init.m:36
         self.size = X.shape                       # init.m:37
+
+    # grid2d.m:45-51: fftz / ifftz are the identity on a 2D grid
+    def fftz(self, phi):
+        return phi
+
+    def ifftz(self, phi):
+        return phi
 
 
 class Grid3D(_GridBase):
@@ -107,6 +136,12 @@ class Grid3D(_GridBase):
         self.mesh = SimpleNamespace(x=X, y=Y, z=Z, x2=X2, y2=Y2)
         self.weight = self.dx * self.dy * self.dz  # init.m:45
         self.size = X.shape                        # init.m:46
+
+    def fftz(self, fun):
+        return sfft.fft(fun, axis=2, workers=WORKERS)
+
+    def ifftz(self, ifun):
+        return sfft.ifft(ifun, axis=2, workers=WORKERS)
 
 
 def grid2d(xmax, nx, ymax, ny):

@@ -324,3 +324,54 @@ def test_2comp_solve_split_is_uncoupled_spinor_step():
         t.init_state = [p.copy() for p in psi]
     pa, pb = a.solve_split(0.01, 10, 2), b.solve_split(0.01, 10, 2)
     assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1])
+
+
+def _lz_spectral(grid, phi):
+    # Lz = -i (x d/dy - y d/dx) (lib/@grid2d/lz.m) with spectral derivatives, numpy.fft
+    kx = grid.kx1[None, :]
+    ky = grid.ky1[:, None]
+    dx = np.fft.ifft(1j * kx * np.fft.fft(phi, axis=1), axis=1)
+    dy = np.fft.ifft(1j * ky * np.fft.fft(phi, axis=0), axis=0)
+    return -1j * (grid.mesh.x * dy - grid.mesh.y * dx)
+
+
+def test_rotating_lap_is_lap_plus_omega_lz():
+    # grid2d/lap.m with omega: the cross terms -kx y omega and +ky x omega are omega*Lz
+    grid = oracle.grid2d(6, 32, 5, 24)
+    rng = np.random.Generator(np.random.PCG64(4))
+    phi = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 3) * (1 + 0.3 * rng.normal(size=grid.size))
+    om = 0.37
+    assert rel_l2(grid.lap(phi, om) - grid.lap(phi), om * _lz_spectral(grid, phi)) < 1e-12
+
+
+@pytest.mark.parametrize("m", [1, -1])
+def test_rotating_frame_vortex_eigenstate_is_stationary(m):
+    # g = 0, V = (x^2+y^2)/2: (x + i m y) e^{-r^2/2} has H = 2 and Lz = m, so in the frame
+    # of lap(phi, omega) its energy is 2 + m omega; solve_split subtracts mu = <H_omega>,
+    # so the state is stationary up to the splitting error O(dt^2)
+    grid = oracle.grid2d(7, 48, 7, 48)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    phi0 = (grid.mesh.x + 1j * m * grid.mesh.y) * np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2)
+    errs = []
+    for dt in (0.01, 0.005):
+        t = GPEtask(grid, V)
+        t.g, t.omega, t.Ntotal = 0.0, 0.4, 1.0
+        t.init_state = grid.normalize(phi0)
+        out = t.solve_split(dt, 20, int(round(1.0 / (20 * dt))))
+        assert abs(t.history["mu"][0] - (2 + m * 0.4)) < 1e-10
+        errs.append(rel_l2(out, t.init_state))
+    assert errs[0] < 1e-4 and 3.0 < errs[0] / errs[1] < 5.0   # second order in dt
+
+
+def test_rotating_frame_itp_keeps_the_gaussian_ground_state():
+    # for omega < 1 the ground state of H + omega Lz in the isotropic trap is still the
+    # Gaussian (Lz = 0), mu = 1
+    grid = oracle.grid2d(6, 32, 6, 32)
+    t = GPEtask(grid, 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2))
+    t.g, t.omega, t.Ntotal = 0.0, 0.5, 1.0
+    t.itp_adjust_stepsize = True
+    phi, MU = t.groundstate_itp(0.01, 1e-7, "rand", nargout=2)
+    assert t.itp_iterations < t.itp_max_iter
+    assert abs(MU[-1] - 1.0) < 1e-4
+    gauss = grid.normalize(np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2) + 0j)
+    assert abs(abs(grid.inner(gauss, phi)) - 1.0) < 1e-6
