@@ -127,6 +127,7 @@ int check_args(gpe_handle* h, const gpe_control_args* a, Ctl* c) {
   if (!a) return fail(h, GPE_EINVAL, "null control args");
   if (h->prm.ncomp != 1) return fail(h, GPE_EINVAL, "the adjoint supports one component");
   if (h->prm.gamma != 0.0) return fail(h, GPE_EINVAL, "the adjoint needs gamma = 0 (exactly invertible steps)");
+  if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the adjoint does not support the rotating frame");
   if (h->adj_nbasis <= 0) return fail(h, GPE_ESTATE, "gpe_control_set has not been called");
   const int nr = h->prm.n_recalc;
   if (a->nsteps <= 0 || a->nsteps % nr || a->steps_per_interval <= 0 || a->steps_per_interval % nr)

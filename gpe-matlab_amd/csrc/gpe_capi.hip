@@ -113,6 +113,13 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     if (hipMalloc(&h->ax[a].keax, n * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "ke");
     if (hipMemcpy(h->ax[a].tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "tw");
     if (hipMemcpy(h->ax[a].keax, ke.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "ke");
+    std::vector<double> k1(n);
+    for (int i = 0; i < n; ++i) k1[i] = h->ax[a].k[i] / (double)n;
+    if (hipMalloc(&h->ax[a].k1ax, n * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "k1");
+    if (hipMalloc(&h->ax[a].xc, n * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM, "xc");
+    if (hipMemcpy(h->ax[a].k1ax, k1.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return bail(GPE_EHIP, "k1");
+    if (hipMemcpy(h->ax[a].xc, h->ax[a].x.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(GPE_EHIP, "xc");
   }
   DevState s{};
   s.scale = 1.0;
@@ -136,7 +143,11 @@ int gpe_destroy(gpe_handle* h) {
   for (int a = 0; a < 3; ++a) {
     (void)hipFree(h->ax[a].tw);
     (void)hipFree(h->ax[a].keax);
+    (void)hipFree(h->ax[a].k1ax);
+    (void)hipFree(h->ax[a].xc);
   }
+  (void)hipFree(h->rtab_x);
+  (void)hipFree(h->rtab_y);
   (void)hipFree(h->cpl);
   (void)hipFree(h->kcol);
   (void)hipFree(h->kax);
@@ -259,11 +270,22 @@ int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
   return GPE_OK;
 }
 
+int gpe_set_omega(gpe_handle* h, double omega) {
+  if (!h) return GPE_EINVAL;
+  if (!std::isfinite(omega)) return fail(h, GPE_EINVAL, "omega must be finite");
+  h->omega = omega;
+  h->begun = false;   // the kinetic tables change: the next solve_split starts with gpe_split_begin
+  return GPE_OK;
+}
+
 int gpe_measure(gpe_handle* h, double out[5]) {
   int rc;
   if ((rc = ensure_ready(h))) return rc;
   if (!out) return GPE_EINVAL;
-  if (h->prm.ncomp == 2) return gpe::spinor_measure(h, out);
+  if (h->prm.ncomp == 2) {
+    if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the rotating frame is supported for one component");
+    return gpe::spinor_measure(h, out);
+  }
   FinParams f = fin_params(h, gpe::FIN_SUMS);
   if ((rc = launch_measure(h, 0, &f))) return rc;
   HIPCHK(h, gpe::launch_finalize(f, h->stream));
@@ -282,7 +304,10 @@ int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out)
   int rc;
   if ((rc = ensure_ready(h))) return rc;
   if (niter_inner < 1) return fail(h, GPE_EINVAL, "niter_inner must be >= 1");
-  if (h->prm.ncomp == 2) return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
+  if (h->prm.ncomp == 2) {
+    if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the rotating frame is supported for one component");
+    return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
+  }
   if ((rc = materialize(h))) return rc;
   double m[5];
   if ((rc = gpe_measure(h, m))) return rc;
@@ -295,7 +320,7 @@ int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out)
   const std::complex<double> dt = cdiv({0.0, ddt}, {1.0, h->prm.gamma});   // 31
   h->dr = dt.real();
   h->di = dt.imag();
-  if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;  // 32
+  if ((rc = build_step_tables(h, dt))) return rc;             // 32 (set_kinop.m)
   // 41: muc = real(inner(phi, applyham(phi)))/NN0
   const double muc = ((m[1] + m[3]) + h->prm.g[0] * m[2]) / h->NN0;
   DevState s;
@@ -374,22 +399,23 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
   double dtmin = std::sqrt(a->eps);
   h->dr = dt;
   h->di = 0.0;
-  if ((rc = build_kinetic(h, {dt, 0.0}, h->kcol, h->kax))) return rc;   // 30
+  if ((rc = build_step_tables(h, {dt, 0.0}))) return rc;       // 30 (set_kinop.m)
+  const bool rot = h->omega != 0.0;
   int i = 0;
   int conv = 1;
   for (;;) {
     ++i;
     // 76 (36 on the first pass) and 40: N from the renormalised density, x N, forward x
-    if ((rc = sweep(h, 0, gpe::S_ITPB, base_params(h, 0, psi, V)))) return rc;
+    if ((rc = sweep(h, 0, rot ? gpe::S_RITPB : gpe::S_ITPB, base_params(h, 0, psi, V)))) return rc;
     if (h->ndims == 3) {
       if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, V)))) return rc;
       if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
       if ((rc = sweep(h, 1, gpe::S_INV, base_params(h, 1, psi, V)))) return rc;
     } else {
-      if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
+      if ((rc = sweep(h, 1, rot ? gpe::S_RKIN : gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
     }
     int nb = 0;
-    if ((rc = sweep(h, 0, gpe::S_ITPA, base_params(h, 0, psi, V), &nb))) return rc;   // 41-44
+    if ((rc = sweep(h, 0, rot ? gpe::S_RITPA : gpe::S_ITPA, base_params(h, 0, psi, V), &nb))) return rc;   // 41-44
     FinParams f = fin_params(h, gpe::FIN_ITP);
     f.cnt[gpe::SL_A] = nb;
     f.dt = dt;
@@ -430,7 +456,7 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
         if (!a->adjust_stepsize || dt < dtmin) break;
         dt = dt / 2;
         h->dr = dt;
-        if ((rc = build_kinetic(h, {dt, 0.0}, h->kcol, h->kax))) return rc;
+        if ((rc = build_step_tables(h, {dt, 0.0}))) return rc;
       }
     }
     if (i >= a->max_iter) {                                        // 78-81

@@ -29,6 +29,8 @@ struct Axis {
   double* keax = nullptr;     // k^2/2/n
   std::vector<double> k;      // wave vectors (grid*/init.m k layout)
   std::vector<double> x;      // coordinates
+  double* k1ax = nullptr;     // device k/n (rotating-frame kinetic sums)
+  double* xc = nullptr;       // device coordinates
 };
 
 }  // namespace gpe_detail
@@ -47,6 +49,9 @@ struct gpe_handle {
   double cpl_re = 0, cpl_im = 0;
   bool cpl_array = false;
   double cpl_max2 = 0;          // max |Omega|^2 of the coupling array (NaN propagates)
+  double omega = 0;             // rotation frequency (GPEtask.m:7), scalar
+  double2* rtab_x = nullptr;    // rotating frame: x half-step table [iy][kx]
+  double2* rtab_y = nullptr;    // rotating frame, 2D: y step table [ix][ky]
   double2* kcol = nullptr;      // kinetic factor per mid-axis column
   double2* kax = nullptr;       // kinetic factor per mid-axis index (incl. 1/npts)
   double2* kcol_sp[2] = {nullptr, nullptr};  // spinor: exp(-kk dt/2) and exp(-kk dt) exp(kk dt/2)
@@ -201,6 +206,15 @@ inline SweepParams base_params(gpe_handle* h, int axis, double2* psi, const doub
   p.g = h->prm.g[0];
   p.dr = h->dr;
   p.di = h->di;
+  if (h->omega != 0.0 && axis < 2) {
+    // lap(phi, omega) (grid2d/lap.m:8-9, grid3d/lap.m:8-10): x pencils carry -kx y omega,
+    // y pencils +ky x omega; the row of a pencil is its y (x pencils) or x (y pencils) index
+    p.trow = (unsigned)(axis == 0 ? h->ny : h->nx);
+    p.rcoord = h->ax[axis == 0 ? 1 : 0].xc;
+    p.rot_w = axis == 0 ? -h->omega : h->omega;
+    p.k1ax = a.k1ax;
+    p.rtab = axis == 0 ? h->rtab_x : h->rtab_y;
+  }
   return p;
 }
 
@@ -271,6 +285,59 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
   HIPCHK(h, hipMemcpyAsync(kax, axv.data(), axv.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return GPE_OK;
+}
+
+// Rotating-frame kinetic tables for complex dt (set_kinop.m:6-7):
+//   x half step  rtab_x[iy][kx] = exp(-(kx^2/2 - kx y omega) dt/2) / nx
+//   3D y/z step  kcol[iy + ny ix] = exp(-(ky^2/2 + ky x omega) dt), kax[kz] = exp(-kz^2/2 dt) / (ny nz)
+//   2D y step    rtab_y[ix][ky] = exp(-(ky^2/2 + ky x omega) dt) / ny
+inline int build_kinetic_rot(gpe_handle* h, std::complex<double> dt) {
+  const double om = h->omega;
+  const int nx = h->nx, ny = h->ny;
+  const std::vector<double>& kx = h->ax[0].k;
+  const std::vector<double>& ky = h->ax[1].k;
+  const std::vector<double>& xs = h->ax[0].x;
+  const std::vector<double>& ys = h->ax[1].x;
+  if (!h->rtab_x) HIPCHK(h, hipMalloc(&h->rtab_x, (size_t)nx * ny * sizeof(double2)));
+  std::vector<double2> tx((size_t)nx * ny);
+  for (int iy = 0; iy < ny; ++iy)
+    for (int ix = 0; ix < nx; ++ix) {
+      const std::complex<double> e = std::exp(-(kx[ix] * kx[ix] / 2 - kx[ix] * ys[iy] * om) * dt / 2.0) / (double)nx;
+      tx[(size_t)iy * nx + ix] = make_double2(e.real(), e.imag());
+    }
+  HIPCHK(h, hipMemcpyAsync(h->rtab_x, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  std::vector<double2> ty((size_t)nx * ny);
+  if (h->ndims == 3) {
+    const int nz = h->nz;
+    const std::vector<double>& kz = h->ax[2].k;
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy) {
+        const std::complex<double> e = std::exp(-(ky[iy] * ky[iy] / 2 + ky[iy] * xs[ix] * om) * dt);
+        ty[(size_t)iy + (size_t)ny * ix] = make_double2(e.real(), e.imag());
+      }
+    std::vector<double2> az(nz);
+    for (int iz = 0; iz < nz; ++iz) {
+      const std::complex<double> e = std::exp(-(kz[iz] * kz[iz] / 2) * dt) / ((double)ny * nz);
+      az[iz] = make_double2(e.real(), e.imag());
+    }
+    HIPCHK(h, hipMemcpyAsync(h->kcol, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->kax, az.data(), az.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  } else {
+    if (!h->rtab_y) HIPCHK(h, hipMalloc(&h->rtab_y, (size_t)nx * ny * sizeof(double2)));
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy) {
+        const std::complex<double> e = std::exp(-(ky[iy] * ky[iy] / 2 + ky[iy] * xs[ix] * om) * dt) / (double)ny;
+        ty[(size_t)ix * ny + iy] = make_double2(e.real(), e.imag());
+      }
+    HIPCHK(h, hipMemcpyAsync(h->rtab_y, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+// The kinetic tables of the current frame.
+inline int build_step_tables(gpe_handle* h, std::complex<double> dt) {
+  return h->omega != 0.0 ? build_kinetic_rot(h, dt) : build_kinetic(h, dt, h->kcol, h->kax);
 }
 
 inline int read_state(gpe_handle* h, DevState* out) {
@@ -353,15 +420,19 @@ inline int run_block_slabs(gpe_handle* h, double time2) {
 }
 
 inline int run_block(gpe_handle* h, double time2) {
-  if (h->ndims == 3 && h->slabs > 1 && h->nz % h->slabs == 0) return run_block_slabs(h, time2);
+  const bool rot = h->omega != 0.0;
+  if (!rot && h->ndims == 3 && h->slabs > 1 && h->nz % h->slabs == 0) return run_block_slabs(h, time2);
   int rc;
   double2* psi = h->psi[0];
   const double* V = h->V[0];
+  // rotating frame: the x sweeps carry the x half steps of ssft_kin_step.m:23,25 around
+  // the phases, and the 2D y step uses the (x, ky) table
+  const int s_open = rot ? gpe::S_ROPEN : gpe::S_OPEN, s_mid = rot ? gpe::S_RMID : gpe::S_MID;
+  const int s_close = rot ? gpe::S_RCLOSEM : gpe::S_CLOSEM, s_kin2d = rot ? gpe::S_RKIN : gpe::S_KIN;
   for (int i = 0; i < h->n_rec; ++i) {
-    const int xseq = i == 0 ? gpe::S_OPEN : gpe::S_MID;
     SweepParams px = base_params(h, 0, psi, V);
     px.reverse = (h->snake & 4) ? 1 : 0;
-    if ((rc = sweep(h, 0, i > 0 ? gpe::S_MID : xseq, px))) return rc;
+    if ((rc = sweep(h, 0, i > 0 ? s_mid : s_open, px))) return rc;
     if (h->ndims == 3) {
       SweepParams py = base_params(h, 1, psi, V);
       py.reverse = (h->snake & 1) ? 1 : 0;
@@ -373,13 +444,13 @@ inline int run_block(gpe_handle* h, double time2) {
       pi.reverse = (h->snake & 2) ? 1 : 0;
       if ((rc = sweep(h, 1, gpe::S_INV, pi))) return rc;
     } else {
-      if ((rc = sweep(h, 1, gpe::S_KIN, base_params(h, 1, psi, V)))) return rc;
+      if ((rc = sweep(h, 1, s_kin2d, base_params(h, 1, psi, V)))) return rc;
     }
   }
   // closing phases fused with the x part of the block-end reductions
   FinParams f = fin_params(h, gpe::FIN_BLOCK);
   int nb = 0;
-  if ((rc = sweep(h, 0, gpe::S_CLOSEM, base_params(h, 0, psi, V), &nb))) return rc;
+  if ((rc = sweep(h, 0, s_close, base_params(h, 0, psi, V), &nb))) return rc;
   f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
   for (int a = 1; a < h->ndims; ++a) {
     if ((rc = sweep(h, a, gpe::S_KE, base_params(h, a, psi, V), &nb))) return rc;

@@ -64,6 +64,7 @@ extern "C" int gpe_itp2(gpe_handle* h, const gpe_itp2_args* a, double* MU, doubl
   if ((rc = ensure_ready(h))) return rc;
   if (!a || !MU || a->max_iter < 1) return fail(h, GPE_EINVAL, "bad two-component ITP arguments");
   if (h->prm.ncomp != 2) return fail(h, GPE_EINVAL, "gpe_itp2 needs two components");
+  if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the rotating frame is supported for one component");
   if (a->mixture && !(a->M1 > 0 && a->M2 > 0)) return fail(h, GPE_EINVAL, "masses M1, M2 must be positive");
   if (!(a->dt > 0) || !std::isfinite(a->dt)) return fail(h, GPE_EINVAL, "dt must be positive");
   const bool mix = a->mixture != 0;

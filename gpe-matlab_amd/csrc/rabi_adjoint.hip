@@ -91,6 +91,7 @@ int check_args(gpe_handle* h, const gpe_rabi_args* a, Plan* c) {
   if (!h->have_params) return fail(h, GPE_ESTATE, "gpe_set_params has not been called");
   if (h->prm.ncomp != 2) return fail(h, GPE_EINVAL, "the Rabi adjoint needs two components");
   if (h->prm.gamma != 0.0) return fail(h, GPE_EINVAL, "the Rabi adjoint needs gamma = 0");
+  if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the Rabi adjoint does not support the rotating frame");
   if (h->prm.nlincpl != 0.0) return fail(h, GPE_EINVAL, "the Rabi adjoint needs nlincpl = 0");
   if (!(h->prm.Ntotal > 0)) return fail(h, GPE_EINVAL, "the Rabi adjoint needs Ntotal > 0");
   if (!std::isfinite(a->ddt) || !std::isfinite(a->omega0) || !std::isfinite(a->k))

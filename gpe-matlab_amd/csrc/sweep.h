@@ -23,7 +23,15 @@ enum Seq : int {
   S_ITPA = 8,   // inverse, x stored real operator N, reduce |psi|^2 (groundstate_itp.m:41-44)
   S_ITPB = 9,   // x mu, N = exp(-dt/2 (g mu^2 n + V)), store N, x N, forward (groundstate_itp.m:76,40)
   S_CLOSEM = 10,  // S_CLOSE, then the S_KEX reductions of the stored state in the same pass
-  S_NSEQ = 11
+  // rotating frame (scalar omega, set_kinop.m:5-7, ssft_kin_step.m:22-25): the x half
+  // steps Kx = forward, x rtab[row(col)][kx], inverse, all in real space between sweeps
+  S_RKIN = 11,    // forward, x rtab[row][i], inverse (2D y step, table over (x, ky))
+  S_ROPEN = 12,   // scale, block-open half phase, Kx
+  S_RMID = 13,    // Kx, full phase, Kx
+  S_RCLOSEM = 14, // Kx, full and closing half phase; block-end sums incl. the x kinetic term
+  S_RITPA = 15,   // Kx, x stored operator N, reduce |psi|^2
+  S_RITPB = 16,   // x mu, N = exp(-dt/2 (g mu^2 n + V)), store N, x N, Kx
+  S_NSEQ = 17
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -67,6 +75,12 @@ struct SweepParams {
   double dr, di;             // complex time step dt = dr + i di (ITP: dr = dt, di = 0)
   int apply_scale;           // S_FWD: multiply by the pending scale st->scale first
   int reverse;               // visit pencil tiles in reverse order (cache reuse between sweeps)
+  // rotating frame
+  const double2* rtab;       // S_R*: kinetic table [row][i], row = col % trow (incl. 1/N of the axis)
+  unsigned trow;             // row modulus of rtab and rcoord
+  const double* k1ax;        // S_KEX/S_KE/S_RCLOSEM: k_i / n of the swept axis
+  const double* rcoord;      // coordinate per row (y for x pencils, x for y pencils)
+  double rot_w;              // kinetic-sum cross term weight: sum (k^2/2 + rot_w coord k) |F psi|^2 / n
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

@@ -14,10 +14,30 @@
 namespace gpe {
 
 template <int SEQ> struct SeqTraits {
-  static constexpr bool LOAD_V =
-      SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM || SEQ == S_KEX || SEQ == S_ITPB;
+  static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM ||
+                                 SEQ == S_KEX || SEQ == S_ITPB || SEQ == S_ROPEN || SEQ == S_RMID ||
+                                 SEQ == S_RCLOSEM || SEQ == S_RITPB;
+  static constexpr bool ROT_KX = SEQ == S_ROPEN || SEQ == S_RMID || SEQ == S_RCLOSEM || SEQ == S_RITPA ||
+                                 SEQ == S_RITPB || SEQ == S_RKIN;
   static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE);
 };
+
+// Kinetic step of one pencil with a per-(row, index) table: forward, multiply,
+// inverse (rtab holds the 1/N of the axis).
+template <class PL, int C, bool COLFAST>
+__device__ __forceinline__ void table_kin(double2 (&v)[PL::E], int t, int c, double* lds, const double2* twl,
+                                          const double2* __restrict__ row) {
+  constexpr int T = PL::T, E = PL::E;
+  fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmul(row[t + T * m], v[m]);
+  fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+}
+
+// Kinetic-energy weight of index i: k_i^2/2/n, plus the rotating-frame cross term.
+__device__ __forceinline__ double ke_weight(const SweepParams& p, int i, double rc) {
+  return p.rot_w != 0.0 ? fma(p.rot_w * rc, p.k1ax[i], p.keax[i]) : p.keax[i];
+}
 
 // One work-group's share of a sweep: pencils tile*C .. tile*C + C-1.  `lds`
 // holds Lds<N, C, COLFAST>::SIZE doubles, `twl` 2N entries (twiddles, then the
@@ -57,6 +77,10 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
 
   const bool has_re = p.dr != 0.0;
   double accA = 0.0, accB = 0.0, accC = 0.0, accK = 0.0;
+  const unsigned trow = p.trow ? cg % p.trow : 0u;
+  const double2* __restrict__ rrow = TR::ROT_KX ? p.rtab + (size_t)trow * N : nullptr;
+  const double rc =
+      (SEQ == S_KEX || SEQ == S_KE || SEQ == S_RCLOSEM) && p.rot_w != 0.0 ? p.rcoord[trow] : 0.0;
 
   if constexpr (SEQ == S_OPEN) {
     // solve_split.m:55-56 with the pending renormalisation (solve_split.m:72-73) folded in
@@ -114,7 +138,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
+      for (int m = 0; m < E; ++m) accK += ke_weight(p, t + T * m, rc) * cabs2(v[m]);
     }
   } else if constexpr (SEQ == S_ITPA) {
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
@@ -136,6 +160,52 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
       v[m] = cscale(cscale(v[m], mu), nl);                         // groundstate_itp.m:52 then 40
     }
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_RKIN) {
+    table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);            // ssft_kin_step.m:24 (2D)
+  } else if constexpr (SEQ == S_ROPEN) {
+    const double s = p.st->scale, mu = p.st->mu;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double2 x = cscale(v[m], s);
+      const double a = -(vv[m] - mu) - p.g * cabs2(x);
+      v[m] = mul_cexp(x, a * p.dr * 0.5, a * p.di * 0.5, has_re);   // solve_split.m:55-56
+    }
+    table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);            // ssft_kin_step.m:23
+  } else if constexpr (SEQ == S_RMID || SEQ == S_RCLOSEM) {
+    table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);            // ssft_kin_step.m:25
+    const double mu = p.st->mu;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double vm = vv[m] - mu;
+      const double a = -vm - p.g * cabs2(v[m]);                  // solve_split.m:60
+      double2 x = mul_cexp(v[m], a * p.dr, a * p.di, has_re);     // solve_split.m:61
+      if constexpr (SEQ == S_RCLOSEM) {
+        const double b = vm + p.g * cabs2(x);                      // solve_split.m:63
+        x = mul_cexp(x, b * p.dr * 0.5, b * p.di * 0.5, has_re);   // solve_split.m:64
+      }
+      v[m] = x;
+    }
+    if constexpr (SEQ == S_RMID) table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);   // next step's 23
+  } else if constexpr (SEQ == S_RITPA) {
+    table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);            // ssft_kin_step.m:25
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double nl = p.nop[base + (unsigned)m * step];
+      v[m] = cscale(v[m], nl);                                     // groundstate_itp.m:42
+      accA += active ? cabs2(v[m]) : 0.0;                          // groundstate_itp.m:44
+    }
+  } else if constexpr (SEQ == S_RITPB) {
+    const double mu = p.st->mu_itp;
+    const double mu2 = mu * mu;
+    const double hdt = -p.dr * 0.5;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double n = cabs2(v[m]);
+      const double nl = exp_cw(hdt * (n * p.g * mu2 + vv[m]));     // groundstate_itp.m:76 (36: mu = 1)
+      if (active) p.nop[base + (unsigned)m * step] = nl;
+      v[m] = cscale(cscale(v[m], mu), nl);                         // groundstate_itp.m:52 then 40
+    }
+    table_kin<PL, C, COLFAST>(v, t, c, lds, twl, rrow);            // ssft_kin_step.m:23
   }
 
   if constexpr (TR::STORE) {
@@ -145,7 +215,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
     }
   }
 
-  if constexpr (SEQ == S_CLOSEM) {
+  if constexpr (SEQ == S_CLOSEM || SEQ == S_RCLOSEM) {
     // block-end reductions of the stored real-space state (solve_split.m:65-77)
     if (active) {
 #pragma unroll
@@ -159,12 +229,12 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
+      for (int m = 0; m < E; ++m) accK += ke_weight(p, t + T * m, rc) * cabs2(v[m]);
     }
   }
 
   // deterministic block reductions
-  if constexpr (SEQ == S_KEX || SEQ == S_CLOSEM) {
+  if constexpr (SEQ == S_KEX || SEQ == S_CLOSEM || SEQ == S_RCLOSEM) {
     const double a = block_sum(accA, red);
     const double b = block_sum(accB, red);
     const double cc = block_sum(accC, red);
@@ -178,7 +248,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   } else if constexpr (SEQ == S_KE) {
     const double k = block_sum(accK, red);
     if (tid == 0) p.partials[p.slot_ke * p.maxwg + wg] = k;
-  } else if constexpr (SEQ == S_ITPA) {
+  } else if constexpr (SEQ == S_ITPA || SEQ == S_RITPA) {
     const double a = block_sum(accA, red);
     if (tid == 0) p.partials[SL_A * p.maxwg + wg] = a;
   }

@@ -62,6 +62,7 @@ SIGNATURES = {
     "gpe_set_state": [_P, C.c_int, _DP],
     "gpe_get_state": [_P, C.c_int, _DP],
     "gpe_set_coupling": [_P, _DP, C.c_int],
+    "gpe_set_omega": [_P, _D],
     "gpe_split_begin": [_P, _D, C.c_int, _DP],
     "gpe_split_outer": [_P, C.c_int, _DP, _DP, _DP],
     "gpe_split_block": [_P, _D, _DP, _DP],
@@ -214,6 +215,9 @@ class Device:
             v = to_colmajor_c128(np.broadcast_to(omega, self.shape))
             self.call("gpe_set_coupling", v.view(np.float64).ctypes.data_as(_DP), 1)
 
+    def set_omega(self, omega):
+        self.call("gpe_set_omega", float(omega))
+
     def measure(self):
         out = np.zeros(5)
         self.call("gpe_measure", dptr(out))
@@ -308,14 +312,15 @@ class Device:
 
     def profile_read(self):
         """{(seq, axis): (total_ms, launches)} of the sweeps since the last read."""
-        ms = np.zeros(33)
-        cnt = np.zeros(33, dtype=np.int32)
-        self.call("gpe_profile_read", dptr(ms), cnt.ctypes.data_as(_IP), 33)
-        return {(i // 3, i % 3): (float(ms[i]), int(cnt[i])) for i in range(33) if cnt[i] > 0}
+        n = 3 * len(SEQ_NAMES)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int32)
+        self.call("gpe_profile_read", dptr(ms), cnt.ctypes.data_as(_IP), n)
+        return {(i // 3, i % 3): (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i] > 0}
 
 
 SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb",
-             10: "closem"}
+             10: "closem", 11: "rkin", 12: "ropen", 13: "rmid", 14: "rclosem", 15: "ritpa", 16: "ritpb"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
 
 

@@ -94,12 +94,15 @@ class GPEtask:
         return self.Vtrap
 
     def _check_omega(self):
-        if not np.isscalar(self.omega) or self.omega != 0:
-            raise NotImplementedError("rotating frame (omega != 0) is not part of the accelerated path")
+        # a scalar omega selects the rotating-frame kinetic step (set_kinop.m:5-7); an
+        # array omega (the Crank-Nicolson Lz branch, ssft_kin_step.m:13-21) is not accelerated
+        if not np.isscalar(self.omega):
+            raise NotImplementedError("an array omega (Crank-Nicolson Lz step) is not part of the accelerated path")
 
     def _set_params(self, Ntotal_eff):
-        self._device().set_params(self._ncomp, self.g, self.gamma, self.decay_rate, Ntotal_eff, self.mu_init,
-                                  self.n_recalc)
+        dev = self._device()
+        dev.set_params(self._ncomp, self.g, self.gamma, self.decay_rate, Ntotal_eff, self.mu_init, self.n_recalc)
+        dev.set_omega(float(self.omega))
 
     def _hist_set(self, key, step, val):
         arr = self.history[key]
@@ -310,9 +313,15 @@ class GPEtaskSpinor(GPEtask):
             return [self.Vtrap[i] + self.Vtd[i](self, time) for i in range(self.ncomp)]
         return self.Vtrap
 
+    def _check_omega(self):
+        if not np.isscalar(self.omega) or self.omega != 0:
+            raise NotImplementedError("the two-component classes run without rotation (omega = 0)")
+
     def _set_params(self, Ntotal_eff):
-        self._device().set_params(2, np.asarray(self.g, dtype=float).reshape(2, 2), self.gamma, self.decay_rate,
-                                  Ntotal_eff, self.mu_init, self.n_recalc, self.nlincpl)
+        dev = self._device()
+        dev.set_params(2, np.asarray(self.g, dtype=float).reshape(2, 2), self.gamma, self.decay_rate,
+                       Ntotal_eff, self.mu_init, self.n_recalc, self.nlincpl)
+        dev.set_omega(0.0)
 
     def _fetch(self):
         d = self._device()

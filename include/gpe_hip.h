@@ -72,6 +72,12 @@ int gpe_set_potential(gpe_handle* h, int comp, const double* V);
 int gpe_set_state(gpe_handle* h, int comp, const double* psi);
 int gpe_get_state(gpe_handle* h, int comp, double* psi);
 
+/* Rotation frequency of the rotating frame (GPEtask.m:7, scalar omega): the kinetic
+ * step becomes the axis-split step of set_kinop.m:5-7 / ssft_kin_step.m:22-25 and
+ * applyham's kinetic term lap(phi, omega) (grid2d/lap.m, grid3d/lap.m).  One
+ * component only.  Takes effect at the next gpe_split_begin / gpe_itp. */
+int gpe_set_omega(gpe_handle* h, double omega);
+
 /* Spinor Rabi coupling (GPEtaskSpinor.m:6): scalar (is_array = 0, omega = {re, im})
  * or space-dependent complex array of npts values (is_array = 1). */
 int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array);
