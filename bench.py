@@ -35,7 +35,8 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBPS = 8000.0
 # algorithmic HBM bytes per grid point per launch of each sweep sequence
 SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40, "kex": 24, "ke": 16,
-             "itpa": 40, "itpb": 48, "closem": 40}
+             "itpa": 40, "itpb": 48, "closem": 40, "rkin": 32, "ropen": 40, "rmid": 40, "rclosem": 40,
+             "ritpa": 40, "ritpb": 48}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 
 
@@ -73,6 +74,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
+    ap.add_argument("--omega", type=float, default=0.0,
+                    help="c2: rotation frequency (rotating-frame kinetic step); reported as a variant")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2"],
@@ -118,6 +121,7 @@ def main():
     log(f"workload built in {time.perf_counter() - t_setup:.1f}s; device {gpe_hip.device_info()}")
     task = gpe_hip.GPEtask(grid, V)
     task.Ntotal, task.g, task.gamma = props["Ntotal"], props["g"], props["gamma"]
+    task.omega = args.omega
     dev = task._device()
     task._set_params(task.Ntotal)
     dev.set_potential(0, V)
@@ -145,6 +149,7 @@ def main():
         O = oracle.grid3d(grid.xmax, grid.nx, grid.ymax, grid.ny, grid.zmax, grid.nz)
         to = oracle.GPEtask(O, V)
         to.Ntotal, to.g, to.gamma = props["Ntotal"], props["g"], props["gamma"]
+        to.omega = args.omega
         to.init_state = psi0
         cpu_ck = {}
 
@@ -226,7 +231,8 @@ def main():
         "data": "synthetic (seeded Thomas-Fermi x exp(i U[0,0.1]) state)",
         "config": {"workload": f"GPEtask 3D drop-in grid3d(12,{args.n},12,{args.n},12,{args.n}) "
                                "V=0.5(X^2+Y^2+16Z^2) Ntotal=1e6 g=0.02 gamma=0.01 n_recalc=10 "
-                               f"solve_split dt=0.002, {ni} inner steps per outer",
+                               f"solve_split dt=0.002, {ni} inner steps per outer"
+                               + (f", rotating frame omega={args.omega}" if args.omega else ""),
                    "grid": [args.n] * 3, "parallelism": "single GPU"},
         "roofline": roofline,
         "step_alg_GBps": round(step_gbps, 1),
