@@ -78,7 +78,7 @@ def main():
                     help="c2: rotation frequency (rotating-frame kinetic step); reported as a variant")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2", "vtd"],
                     help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
                          "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
@@ -105,7 +105,8 @@ def main():
               "the multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
     if args.workload != "c2":
-        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "itp2": bench_itp2}[args.workload](args)
+        {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "itp2": bench_itp2,
+         "vtd": bench_vtd}[args.workload](args)
         return
 
     import gpe_hip
@@ -580,6 +581,47 @@ def bench_itp2(args):
                                             "nargout=1, fixed iteration count", "grid": [n] * 3},
           iter_alg_GBps=round(gbps, 1), iter_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
           alg_bytes_per_point=bpt, parity_rel_l2_vs_oracle_3_iterations=parity)
+
+
+def bench_vtd(args):
+    """The 256^3 workload with a time-dependent stirring potential
+    Vtd = 5 sin(10 t) exp(-((X-3)^2+Y^2)/2) + 0.05 cos(4 t) (X^2 - Y^2), evaluated per
+    block (every 10 steps): on the device from a resident basis (BasisVtd), and on the
+    host with a full-grid upload per block (the same function as a plain Vtd)."""
+    import gpe_hip
+    from gpe_hip import workloads
+    grid, V, psi0, props = workloads.c2_state(n=args.n)
+    X, Y = grid.mesh.x, grid.mesh.y
+    bv = gpe_hip.BasisVtd([np.exp(-((X - 3.0) ** 2 + Y ** 2) / 2), X ** 2 - Y ** 2],
+                          lambda task, t: [5.0 * np.sin(10.0 * t), 0.05 * np.cos(4.0 * t)])
+    # each solve_split call carries fixed host work (state upload, the basis upload,
+    # the t = 0 potential); the per-step rate is taken from two runs of different
+    # length, (n2 - n1) / (t2 - t1)
+    n1, n2 = 100, 100 + max(100, args.steps)
+    rates = {}
+    for mode in ("device", "host"):
+        t = gpe_hip.GPEtask(grid, V)
+        t.Ntotal, t.g, t.gamma = props["Ntotal"], props["g"], props["gamma"]
+        t.Vtd = bv if mode == "device" else (lambda task, tt: bv(task, tt))
+        el = {}
+        for nsteps in (50, n1, n2):
+            t.init_state = psi0
+            t.current_iter = 0
+            t.current_time = 0.0
+            sync()
+            t0 = time.perf_counter()
+            t.solve_split(0.002, 50, nsteps // 50)
+            sync()
+            el[nsteps] = time.perf_counter() - t0
+        rates[mode] = (n2 - n1) / (el[n2] - el[n1])
+        log(f"vtd {mode}: {rates[mode]:.1f} steps/s (runs of {n1} and {n2} steps: {el[n1]:.3f}s, {el[n2]:.3f}s)")
+    steps = n2 - n1
+    _line("split-steps/s with a time-dependent potential (256^3, V rebuilt per 10-step block)",
+          round(rates["device"], 2), "split-steps/s", steps, 50, steps / rates["device"], True,
+          {"workload": "GPEtask 3D 256^3 (as the default workload) + Vtd = 5 sin(10t) G_stir + 0.05 cos(4t) "
+                       "(X^2-Y^2), solve_split dt=0.002, n_recalc=10", "grid": [args.n] * 3},
+          host_evaluated_vtd_steps_per_s=round(rates["host"], 2),
+          device_over_host=round(rates["device"] / rates["host"], 2))
 
 
 if __name__ == "__main__":

@@ -166,6 +166,9 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->adj_grad);
   (void)hipFree(h->kcol_c);
   (void)hipFree(h->kax_c);
+  for (double* b : h->Vmu) (void)hipFree(b);
+  (void)hipFree(h->vtd_G);
+  (void)hipFree(h->vtd_V0);
   (void)hipFree(h->itp2_P);
   for (double* b : h->itp2_hist) (void)hipFree(b);
   for (int i = 0; i < 2; ++i) (void)hipFree(h->rb_lam[i]);
@@ -216,6 +219,20 @@ int gpe_set_potential(gpe_handle* h, int comp, const double* V) {
   HIPCHK(h, hipMemcpyAsync(h->V[comp], V, h->npts * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_V[comp] = true;
+  return GPE_OK;
+}
+
+int gpe_set_potential_mu(gpe_handle* h, int comp, const double* V) {
+  if (!h || comp < 0 || comp > 1) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
+  if (!V) {
+    h->use_vmu = false;
+    return GPE_OK;
+  }
+  HIPCHK(h, hipSetDevice(h->device));
+  if (!h->Vmu[comp]) HIPCHK(h, hipMalloc(&h->Vmu[comp], h->npts * sizeof(double)));
+  HIPCHK(h, hipMemcpyAsync(h->Vmu[comp], V, h->npts * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->use_vmu = true;
   return GPE_OK;
 }
 

@@ -86,6 +86,13 @@ struct gpe_handle {
   int adj_gradcap = 0;
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
+  // two-component block-end reductions: potential at task.current_time (gpe_set_potential_mu)
+  double* Vmu[2] = {nullptr, nullptr};
+  bool use_vmu = false;
+  // device-evaluated time-dependent potential (gpe_vtd_set / gpe_vtd_coeffs)
+  double* vtd_G = nullptr;        // nbasis x npts basis fields
+  double* vtd_V0 = nullptr;       // ncomp x npts trap potentials
+  int vtd_nb = 0, vtd_nc = 0;
   // two-component ITP (gpe_itp2)
   double* itp2_P = nullptr;       // stored phase factors, 2 x npts
   double* itp2_hist[3] = {nullptr, nullptr, nullptr};   // MU, MU2, EE device histories

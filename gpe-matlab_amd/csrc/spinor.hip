@@ -90,6 +90,10 @@ int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled) {
   int rc, nb = 0;
   gpe::SpinorParams p = spinor_params(h);
   p.apply_sc = scaled ? 1 : 0;
+  if (h->use_vmu && h->Vmu[0] && h->Vmu[1]) {   // applyham at task.current_time (see gpe_set_potential_mu)
+    p.V1 = h->Vmu[0];
+    p.V2 = h->Vmu[1];
+  }
   if ((rc = spinor_sweep(h, gpe::SP_KEX, p, &nb))) return rc;
   for (int s : {gpe::SL_A, gpe::SL_B, gpe::SL_C, gpe::SL_C12, gpe::SL_C2, gpe::SL_X, gpe::SL_KX, gpe::SL_KX2})
     f->cnt[s] = nb;

@@ -63,6 +63,9 @@ SIGNATURES = {
     "gpe_get_state": [_P, C.c_int, _DP],
     "gpe_set_coupling": [_P, _DP, C.c_int],
     "gpe_set_omega": [_P, _D],
+    "gpe_vtd_set": [_P, C.c_int, _DP, _DP],
+    "gpe_vtd_coeffs": [_P, _DP, C.c_int],
+    "gpe_set_potential_mu": [_P, C.c_int, _DP],
     "gpe_split_begin": [_P, _D, C.c_int, _DP],
     "gpe_split_outer": [_P, C.c_int, _DP, _DP, _DP],
     "gpe_split_block": [_P, _D, _DP, _DP],
@@ -214,6 +217,22 @@ class Device:
         else:
             v = to_colmajor_c128(np.broadcast_to(omega, self.shape))
             self.call("gpe_set_coupling", v.view(np.float64).ctypes.data_as(_DP), 1)
+
+    def vtd_set(self, basis, vtraps):
+        G = np.concatenate([to_colmajor_f64(np.broadcast_to(b, self.shape)) for b in basis])
+        V0 = np.concatenate([to_colmajor_f64(np.broadcast_to(v, self.shape)) for v in vtraps])
+        self.call("gpe_vtd_set", len(basis), dptr(G), dptr(V0))
+
+    def vtd_coeffs(self, c, target=0):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.call("gpe_vtd_coeffs", dptr(c), int(target))
+
+    def set_potential_mu(self, comp, V):
+        if V is None:
+            self.call("gpe_set_potential_mu", comp, None)
+            return
+        v = to_colmajor_f64(np.broadcast_to(V, self.shape))
+        self.call("gpe_set_potential_mu", comp, dptr(v))
 
     def set_omega(self, omega):
         self.call("gpe_set_omega", float(omega))

@@ -15,6 +15,7 @@ import warnings
 import numpy as np
 
 from . import _lib
+from .vtd import BasisVtd
 
 _UNSET = object()
 
@@ -231,6 +232,9 @@ class GPEtask:
         if static and Vb is not self.Vtrap:
             dev.set_potential(0, self.Vtrap)
         dt_outer = ddt * niter_inner
+        device_vtd = isinstance(self.Vtd, BasisVtd)
+        if device_vtd:
+            dev.vtd_set(self.Vtd.basis, [self.Vtrap])
         for j in range(start + 1, niter_outer + 1):             # 49
             if static:
                 time2, muc, ncur = dev.split_outer(j)
@@ -238,7 +242,10 @@ class GPEtask:
                 time = (j - 1) * dt_outer
                 for jj in range(1, niter_inner // n_rec + 1):
                     time2 = time + (jj - 1) * ddt * n_rec
-                    dev.set_potential(0, self.getVtotal(time2))   # 54
+                    if device_vtd and not (time2 == 0 and np.size(self.V0) > 1):
+                        dev.vtd_coeffs(self.Vtd.amplitudes(self, time2), 0)   # 54 on the device
+                    else:
+                        dev.set_potential(0, self.getVtotal(time2))   # 54
                     muc, ncur = dev.split_block(time2)
             self.ext_callback(j, time2, muc, ncur)              # 80
         return self.current_state
@@ -387,19 +394,37 @@ class GPEtaskSpinor(GPEtask):
         n_rec = self.n_recalc
         dt_outer = ddt * niter_inner
         cpl = self.coupling
-        for j in range(start + 1, niter_outer + 1):
-            if self.coupling is not cpl:                       # callbacks may change it (61-65)
-                cpl = self.coupling
-                self._upload_coupling()
-            if static:
-                time2, mu, ncur = dev.split_outer(j)
-            else:
-                time = (j - 1) * dt_outer
-                for jj in range(1, niter_inner // n_rec + 1):
-                    time2 = time + (jj - 1) * ddt * n_rec
-                    self._upload_potential(self.getVtotal(time2))
-                    mu, ncur = dev.split_block(time2)
-            self.ext_callback(j, time2, mu, ncur)
+        device_vtd = isinstance(self.Vtd, BasisVtd)
+        if device_vtd:
+            dev.vtd_set(self.Vtd.basis, list(self.Vtrap))
+        try:
+            for j in range(start + 1, niter_outer + 1):
+                if self.coupling is not cpl:                   # callbacks may change it (61-65)
+                    cpl = self.coupling
+                    self._upload_coupling()
+                if static:
+                    time2, mu, ncur = dev.split_outer(j)
+                else:
+                    # the block-end mu applies applyham at task.current_time (109-114)
+                    tc = self.current_time
+                    if device_vtd and not (tc == 0 and np.size(self.V0) > 1):
+                        dev.vtd_coeffs(self.Vtd.amplitudes(self, tc), 1)
+                    else:
+                        Vc = self.getVtotal(tc)
+                        dev.set_potential_mu(0, Vc[0])
+                        dev.set_potential_mu(1, Vc[1])
+                    time = (j - 1) * dt_outer
+                    for jj in range(1, niter_inner // n_rec + 1):
+                        time2 = time + (jj - 1) * ddt * n_rec
+                        if device_vtd and not (time2 == 0 and np.size(self.V0) > 1):
+                            dev.vtd_coeffs(self.Vtd.amplitudes(self, time2))
+                        else:
+                            self._upload_potential(self.getVtotal(time2))
+                        mu, ncur = dev.split_block(time2)
+                self.ext_callback(j, time2, mu, ncur)
+        finally:
+            if not static:
+                dev.set_potential_mu(0, None)
         return self.current_state
 
     # two-component imaginary time ------------------------------------------------

@@ -67,6 +67,14 @@ int gpe_set_params(gpe_handle* h, const gpe_params* p);
  * GPEtaskSpinor.m:24-44).  V is real float64, npts values. */
 int gpe_set_potential(gpe_handle* h, int comp, const double* V);
 
+/* Potential read by the block-end chemical-potential reductions of the
+ * two-component solve_split.  Those apply applyham(phi) without a time argument
+ * (lib/@GPEtaskSpinor/solve_split.m:109-114), i.e. at task.current_time
+ * (GPEtaskSpinor.m:56-60), which with a time-dependent potential is the time of
+ * the last callback rather than the block time.  V = NULL returns the reductions
+ * to the step potential. */
+int gpe_set_potential_mu(gpe_handle* h, int comp, const double* V);
+
 /* init_state / current_state upload and download (GPEtask.m:18-19, 119).  The
  * download applies a pending renormalisation (solve_split.m:72). */
 int gpe_set_state(gpe_handle* h, int comp, const double* psi);
@@ -77,6 +85,18 @@ int gpe_get_state(gpe_handle* h, int comp, double* psi);
  * applyham's kinetic term lap(phi, omega) (grid2d/lap.m, grid3d/lap.m).  One
  * component only.  Takes effect at the next gpe_split_begin / gpe_itp. */
 int gpe_set_omega(gpe_handle* h, double omega);
+
+/* Device-evaluated time-dependent potential (no reference counterpart; replaces the
+ * per-block host evaluation and upload of getVtotal(time2) = Vtrap + Vtd(task, time2),
+ * GPEtask.m:62-71 / GPEtaskSpinor.m:24-44, solve_split.m:53-54, for a Vtd that is a
+ * linear combination of fixed fields).  gpe_vtd_set stores nbasis fields G (nbasis x
+ * npts) and the trap potential of each component (ncomp x npts); gpe_vtd_coeffs
+ * rebuilds every component's potential on the device as
+ * Vtrap_j + (c_0 G_0 + c_1 G_1 + ...), summed in that order.  At most 16 fields. */
+int gpe_vtd_set(gpe_handle* h, int nbasis, const double* G, const double* Vtrap);
+/* target 0: the step potential (gpe_set_potential); 1: the reduction potential
+ * (gpe_set_potential_mu) */
+int gpe_vtd_coeffs(gpe_handle* h, const double* c, int target);
 
 /* Spinor Rabi coupling (GPEtaskSpinor.m:6): scalar (is_array = 0, omega = {re, im})
  * or space-dependent complex array of npts values (is_array = 1). */
