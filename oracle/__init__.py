@@ -18,4 +18,4 @@ data), so end-to-end MATLAB trajectories themselves remain pinned only through
 those checks.
 """
 from .grids import Grid2D, Grid3D, grid2d, grid3d  # noqa: F401
-from .gpetask import GPEtask, GPEtask2comp, GPEtaskSpinor  # noqa: F401
+from .gpetask import GPEtask, GPEtask2comp, GPEtaskSpinor, SGPEtask  # noqa: F401

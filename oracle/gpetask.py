@@ -664,3 +664,71 @@ class GPEtask2comp(GPEtaskSpinor):
         self.init_state = phi                                    # 155-156
         self.current_state = phi
         return out[0] if nargout == 1 else tuple(out)
+
+
+class SGPEtask(GPEtask):
+    """lib/@SGPEtask (SGPEtask.m:1-47): the stochastic GPE (omega = 0).
+
+    The complex Gaussian noise of solve_split.m:59 comes from the counter-based
+    generator of ``oracle/philox.py`` (seed ``noise_seed``, one counter per global
+    step and grid point) instead of MATLAB's ``randn`` stream; the global step of
+    outer iteration j, block jj, inner step i is (j-1)*niter_inner + (jj-1)*n_rec + i-1.
+    """
+
+    def __init__(self, grid, trappot, seed=0):
+        super().__init__(grid, trappot, seed)
+        self.T = 0.0                 # SGPEtask.m:5
+        self.snapshots = []          # SGPEtask.m:6
+        self.ss_maxlength = 100      # SGPEtask.m:7
+        self.ecut = 0.0              # SGPEtask.m:8
+        self.noise_seed = 0
+
+    def ext_callback(self, phi, step, time, mu, n):
+        # SGPEtask.m:20-44: as GPEtask's, returning the user callback's text
+        return super().ext_callback(phi, step, time, mu, n)
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """SGPEtask/solve_split.m:1-81."""
+        from .philox import complex_normals
+        self._check_omega()                                      # 61-68 (Crank-Nicolson Lz) not restated
+        grid = self.grid
+        g = self.g
+        gam = self.gamma
+        n_rec = self.n_recalc
+        start = self.current_iter
+        dt = ddt * 1j / (1 + 1j * gam)                           # 29
+        phi = self.current_state if start > 0 else self.init_state   # 31-35
+        phi = np.array(phi, dtype=np.complex128)
+        tmp2 = np.real(phi * np.conj(phi))                       # 37
+        if self.ecut > 0 and self.T > 0:                         # 39-44
+            mask = grid.kk < self.ecut
+            ekk = np.exp(-grid.kk * dt) * mask
+        else:
+            ekk = np.exp(-grid.kk * dt)
+        dt_outer = ddt * niter_inner                             # 45
+        npts = phi.size
+        variance = np.sqrt(self.T * gam * ddt / grid.weight)     # 47
+        time2 = 0.0
+        ncur = None
+        for j in range(start + 1, niter_outer + 1):              # 49
+            mu = self.mu_init                                    # 50
+            time = (j - 1) * dt_outer
+            for jj in range(1, int(np.floor(niter_inner / n_rec)) + 1):   # 52
+                time2 = time + (jj - 1) * ddt * n_rec
+                VV = self.getVtotal(time2)                       # 54
+                phi = np.exp((mu - VV - g * tmp2) * dt / 2) * phi   # 55
+                for i in range(1, n_rec + 1):                    # 57-70
+                    step = (j - 1) * niter_inner + (jj - 1) * n_rec + (i - 1)
+                    re, im = complex_normals(npts, step, self.noise_seed)
+                    # 59: randn(sz) + 1i*randn(sz), column-major point order
+                    noise = (re + 1j * im).reshape(phi.shape, order="F")
+                    phi = phi + variance * noise
+                    phi = grid.ifft(ekk * grid.fft(phi))         # 60
+                    phi = np.exp((mu - VV - g * phi * np.conj(phi)) * dt) * phi   # 69
+                phi = np.exp((VV - mu + g * phi * np.conj(phi)) * dt / 2) * phi   # 72
+                tmp2 = np.real(phi * np.conj(phi))               # 73
+                ncur = grid.integrate(tmp2)                      # 75
+            err = self.ext_callback(phi, j, time2, mu, ncur)     # 77-80
+            if err == "TERM":
+                break
+        return phi

@@ -375,3 +375,65 @@ def test_rotating_frame_itp_keeps_the_gaussian_ground_state():
     assert abs(MU[-1] - 1.0) < 1e-4
     gauss = grid.normalize(np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2) + 0j)
     assert abs(abs(grid.inner(gauss, phi)) - 1.0) < 1e-6
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10
+    from oracle.philox import philox4x32_10
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in kat:
+        out = philox4x32_10(*[np.uint32(c) for c in ctr], *[np.uint32(k) for k in key])
+        assert tuple(int(o) for o in out) == want
+
+
+def test_complex_normals_statistics():
+    from oracle.philox import complex_normals
+    n = 1 << 20
+    re, im = complex_normals(n, 12345, 7)
+    for x in (re, im):
+        assert abs(x.mean()) < 5 / np.sqrt(n)
+        assert abs(x.var() - 1) < 5 * np.sqrt(2 / n)
+    assert abs(np.mean(re * im)) < 5 / np.sqrt(n)
+    re2, _ = complex_normals(n, 12346, 7)
+    assert abs(np.mean(re * re2)) < 5 / np.sqrt(n)       # neighbouring steps are independent
+
+
+def _sgpe(grid, V, T, g=0.3, gamma=0.05, ecut=0.0, mu=2.0, seed=11):
+    t = oracle.SGPEtask(grid, V)
+    t.g, t.gamma, t.T, t.ecut, t.mu_init, t.noise_seed = g, gamma, T, ecut, mu, seed
+    r2 = grid.mesh.x ** 2 + grid.mesh.y ** 2
+    t.init_state = np.exp(-r2 / 2) * (1 + 0.1j * grid.mesh.x)
+    return t
+
+
+def test_sgpe_at_zero_temperature_is_the_damped_gpe_with_fixed_mu():
+    grid = oracle.grid2d(6, 32, 6, 32)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    s = _sgpe(grid, V, T=0.0)
+    ref = GPEtask(grid, V)
+    ref.g, ref.gamma, ref.mu_init = 0.3, 0.05, 2.0        # mu_init > 0: no renormalisation
+    ref.init_state = s.init_state.copy()
+    assert rel_l2(s.solve_split(0.01, 20, 3), ref.solve_split(0.01, 20, 3)) < 1e-13
+
+
+def test_sgpe_noise_response_is_linear_in_sqrt_temperature():
+    # g = 0: the step is linear, so psi(T) - psi(0) scales with sqrt(T gamma ddt / weight)
+    grid = oracle.grid2d(6, 32, 6, 32)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    out = {T: _sgpe(grid, V, T, g=0.0).solve_split(0.01, 10, 2) for T in (0.0, 0.5, 2.0)}
+    d1, d4 = out[0.5] - out[0.0], out[2.0] - out[0.0]
+    assert rel_l2(d4, 2 * d1) < 1e-12
+    assert np.linalg.norm(d1) > 1e-3 * np.linalg.norm(out[0.0])
+
+
+def test_sgpe_energy_cutoff_empties_the_masked_modes():
+    # V = 0, g = 0: every inner step ends with the masked kinetic factor followed by
+    # uniform phases, so no power is left at kk >= ecut (up to the FFT round trip)
+    grid = oracle.grid2d(6, 32, 6, 32)
+    s = _sgpe(grid, 0.0 * grid.mesh.x, T=1.0, g=0.0, ecut=4.0)
+    phi = s.solve_split(0.01, 10, 1)
+    spec = np.abs(grid.fft(phi)) ** 2
+    assert spec[grid.kk >= 4.0].max() < 1e-25 * spec.max() and spec[grid.kk < 4.0].max() > 0.0
