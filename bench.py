@@ -78,7 +78,7 @@ def main():
                     help="c2: rotation frequency (rotating-frame kinetic step); reported as a variant")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2", "vtd"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2", "vtd", "sgpe"],
                     help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
                          "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
@@ -106,7 +106,7 @@ def main():
         sys.exit(2)
     if args.workload != "c2":
         {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "itp2": bench_itp2,
-         "vtd": bench_vtd}[args.workload](args)
+         "vtd": bench_vtd, "sgpe": bench_sgpe}[args.workload](args)
         return
 
     import gpe_hip
@@ -622,6 +622,52 @@ def bench_vtd(args):
                        "(X^2-Y^2), solve_split dt=0.002, n_recalc=10", "grid": [args.n] * 3},
           host_evaluated_vtd_steps_per_s=round(rates["host"], 2),
           device_over_host=round(rates["device"] / rates["host"], 2))
+
+
+def bench_sgpe(args):
+    """Stochastic GPE (SGPEtask.solve_split) on the 256^3 grid of the default workload:
+    T = 20, gamma = 0.01, ecut = 60, mu_init = 75; noise from the counter-based stream."""
+    import gpe_hip
+    from gpe_hip import workloads
+    grid, V, psi0, props = workloads.c2_state(n=args.n)
+    t = gpe_hip.SGPEtask(grid, V)
+    t.g, t.gamma, t.T, t.ecut, t.mu_init, t.noise_seed = props["g"], 0.01, 20.0, 60.0, 75.0, 3
+    t.init_state = psi0
+    parity = None
+    if not args.no_parity:
+        import oracle
+        O = oracle.grid3d(12, args.n, 12, args.n, 12, args.n)
+        to = oracle.SGPEtask(O, V)
+        to.g, to.gamma, to.T, to.ecut, to.mu_init, to.noise_seed = props["g"], 0.01, 20.0, 60.0, 75.0, 3
+        to.init_state = psi0
+        pg = t.solve_split(0.002, 10, 1)
+        c0 = time.perf_counter()
+        po = to.solve_split(0.002, 10, 1)
+        cel = time.perf_counter() - c0
+        parity = float(np.linalg.norm(pg - po) / np.linalg.norm(po))
+        log(f"sgpe parity (10 steps) rel-L2 {parity:.2e}; oracle {cel:.1f}s")
+    dev = t._device()
+    t._set_params(1.0)
+    dev.set_potential(0, V)
+    dev.set_state(0, psi0)
+    dev.sgpe_begin(0.002, 50, t.T, t.ecut, t.noise_seed)
+    dev.sgpe_outer(1)                                      # warm-up
+    nout = max(1, args.steps // 50)
+    sync()
+    t0 = time.perf_counter()
+    for j in range(2, 2 + nout):
+        dev.sgpe_outer(j)
+    sync()
+    el = time.perf_counter() - t0
+    steps = nout * 50
+    rate = steps / el
+    gbps = STEP_BYTES_3D * args.n ** 3 * rate / 1e9
+    log(f"sgpe: {steps} steps in {el:.3f}s -> {rate:.1f} steps/s, {gbps:.0f} GB/s algorithmic")
+    _line("stochastic GPE split-steps/s (SGPEtask.solve_split 3D 256^3)", round(rate, 2), "split-steps/s", steps, 50,
+          el, True, {"workload": "SGPEtask 3D grid3d(12,256,...) V=0.5(X^2+Y^2+16Z^2), g=0.02, gamma=0.01, T=20, "
+                                 "ecut=60, mu_init=75, dt=0.002, n_recalc=10", "grid": [args.n] * 3},
+          step_alg_GBps=round(gbps, 1), step_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
+          parity_rel_l2_vs_oracle_10_steps=parity)
 
 
 if __name__ == "__main__":

@@ -86,6 +86,12 @@ struct gpe_handle {
   int adj_gradcap = 0;
   double2* kcol_c = nullptr;      // conjugate kinetic tables (backward steps)
   double2* kax_c = nullptr;
+  // stochastic GPE (gpe_sgpe_*)
+  bool sg_begun = false, sg_mask = false;
+  double sg_var = 0, sg_ecut = 0;
+  unsigned long long sg_seed = 0;
+  double* sg_kkcol = nullptr;     // kx^2 (+ ky^2) per mid-axis column
+  double* sg_kkax = nullptr;      // k^2 per mid-axis index
   // two-component block-end reductions: potential at task.current_time (gpe_set_potential_mu)
   double* Vmu[2] = {nullptr, nullptr};
   bool use_vmu = false;

@@ -1,0 +1,135 @@
+// Stochastic GPE: SGPEtask.solve_split (lib/@SGPEtask/solve_split.m:1-81), omega = 0.
+//
+// Per block (52-76): block-open half phase with the stored density, n_rec inner
+// steps (noise of step i added in real space -- 59 --, kinetic factor exp(-kk dt),
+// masked to kk < ecut when ecut > 0 and T > 0 -- 39-44, 60 --, full phase -- 69 --),
+// closing half phase (72) and ncur = integrate(|psi|^2) (73-75).  mu = mu_init is
+// fixed and the state is never renormalised.  The noise of global step s at grid
+// point p is the complex normal variate of Philox counter (p, s) under the task's
+// noise seed (philox_dev.h), so a run is reproducible and resumable.
+#include <cmath>
+#include <vector>
+
+#include "handle.h"
+
+using namespace gpe_detail;
+
+namespace {
+
+SweepParams sg_params(gpe_handle* h, int axis) {
+  SweepParams p = base_params(h, axis, h->psi[0], h->V[0]);
+  p.noise_var = h->sg_var;
+  p.noise_seed = h->sg_seed;
+  p.kkcol = h->sg_kkcol;
+  p.kkax = h->sg_kkax;
+  p.ecut = h->sg_ecut;
+  return p;
+}
+
+int sg_block(gpe_handle* h, long long step0, double* ncur) {
+  int rc;
+  const bool noisy = h->sg_var != 0.0;
+  const int kin = h->sg_mask ? gpe::S_KINM : gpe::S_KIN;
+  for (int i = 0; i < h->n_rec; ++i) {
+    SweepParams px = sg_params(h, 0);
+    px.noise_step = (unsigned long long)(step0 + i);
+    const int xs = i == 0 ? (noisy ? gpe::S_SOPEN : gpe::S_OPEN) : (noisy ? gpe::S_SMID : gpe::S_MID);
+    if ((rc = sweep(h, 0, xs, px))) return rc;
+    if (h->ndims == 3) {
+      if ((rc = sweep(h, 1, gpe::S_FWD, sg_params(h, 1)))) return rc;
+      if ((rc = sweep(h, 2, kin, sg_params(h, 2)))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_INV, sg_params(h, 1)))) return rc;
+    } else {
+      if ((rc = sweep(h, 1, kin, sg_params(h, 1)))) return rc;
+    }
+  }
+  int nb = 0;
+  FinParams f = fin_params(h, gpe::FIN_SUMS);
+  if ((rc = sweep(h, 0, gpe::S_CLOSEM, sg_params(h, 0), &nb))) return rc;   // 72-75
+  f.cnt[gpe::SL_A] = nb;
+  HIPCHK(h, gpe::launch_finalize(f, h->stream));
+  if (ncur) {
+    DevState s;
+    if ((rc = read_state(h, &s))) return rc;
+    *ncur = s.sA * h->w;
+  }
+  return GPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!a) return fail(h, GPE_EINVAL, "null SGPE arguments");
+  if (h->prm.ncomp != 1) return fail(h, GPE_EINVAL, "the stochastic GPE has one component");
+  if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the stochastic GPE runs without rotation (omega = 0)");
+  if (a->niter_inner < 1 || h->prm.n_recalc < 1) return fail(h, GPE_EINVAL, "niter_inner and n_recalc must be >= 1");
+  if (!(a->T >= 0) || !std::isfinite(a->T) || !std::isfinite(a->ecut)) return fail(h, GPE_EINVAL, "bad T or ecut");
+  if ((rc = materialize(h))) return rc;
+  h->n_rec = h->prm.n_recalc;                          // no n_rec fallback (52 truncates)
+  h->niter_inner = a->niter_inner;
+  h->ddt = a->ddt;
+  const std::complex<double> dt = cdiv({0.0, a->ddt}, {1.0, h->prm.gamma});   // 29
+  h->dr = dt.real();
+  h->di = dt.imag();
+  if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;                 // 41 / 43
+  h->sg_mask = a->ecut > 0 && a->T > 0;                                         // 39
+  h->sg_ecut = a->ecut;
+  h->sg_var = std::sqrt(a->T * h->prm.gamma * a->ddt / h->w);                 // 47
+  h->sg_seed = a->seed;
+  if (h->sg_mask) {
+    const int ma = mid_axis(h);
+    const Axis& m = h->ax[ma];
+    std::vector<double> col((size_t)m.ncols), ax(m.n);
+    if (h->ndims == 3) {
+      for (int ix = 0; ix < h->nx; ++ix)
+        for (int iy = 0; iy < h->ny; ++iy)
+          col[(size_t)iy + (size_t)h->ny * ix] = h->ax[0].k[ix] * h->ax[0].k[ix] + h->ax[1].k[iy] * h->ax[1].k[iy];
+    } else {
+      for (int ix = 0; ix < h->nx; ++ix) col[ix] = h->ax[0].k[ix] * h->ax[0].k[ix];
+    }
+    for (int i = 0; i < m.n; ++i) ax[i] = m.k[i] * m.k[i];
+    if (!h->sg_kkcol) HIPCHK(h, hipMalloc(&h->sg_kkcol, col.size() * sizeof(double)));
+    if (!h->sg_kkax) HIPCHK(h, hipMalloc(&h->sg_kkax, ax.size() * sizeof(double)));
+    HIPCHK(h, hipMemcpy(h->sg_kkcol, col.data(), col.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->sg_kkax, ax.data(), ax.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  DevState s;
+  if ((rc = read_state(h, &s))) return rc;
+  s.mu = h->prm.mu_init;                                // 50
+  s.scale = 1.0;
+  if ((rc = write_state(h, s))) return rc;
+  h->sg_begun = true;
+  return GPE_OK;
+}
+
+int gpe_sgpe_block(gpe_handle* h, long long step0, double* ncur) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!h->sg_begun) return fail(h, GPE_ESTATE, "gpe_sgpe_begin has not been called");
+  if (step0 < 0) return fail(h, GPE_EINVAL, "negative step");
+  return sg_block(h, step0, ncur);
+}
+
+int gpe_sgpe_outer(gpe_handle* h, int j, double* time2_out, double* ncur) {
+  int rc;
+  if ((rc = ensure_ready(h))) return rc;
+  if (!h->sg_begun) return fail(h, GPE_ESTATE, "gpe_sgpe_begin has not been called");
+  if (j < 1) return fail(h, GPE_EINVAL, "j must be >= 1");
+  const double dt_outer = h->ddt * h->niter_inner;
+  const double time = (double)(j - 1) * dt_outer;       // 51
+  double time2 = time;
+  const int nblocks = h->niter_inner / h->n_rec;
+  for (int jj = 1; jj <= nblocks; ++jj) {
+    time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
+    const long long step0 = (long long)(j - 1) * h->niter_inner + (long long)(jj - 1) * h->n_rec;
+    if ((rc = sg_block(h, step0, jj == nblocks ? ncur : nullptr))) return rc;
+  }
+  if (time2_out) *time2_out = time2;
+  return GPE_OK;
+}
+
+}  // extern "C"

@@ -31,7 +31,12 @@ enum Seq : int {
   S_RCLOSEM = 14, // Kx, full and closing half phase; block-end sums incl. the x kinetic term
   S_RITPA = 15,   // Kx, x stored operator N, reduce |psi|^2
   S_RITPB = 16,   // x mu, N = exp(-dt/2 (g mu^2 n + V)), store N, x N, Kx
-  S_NSEQ = 17
+  // stochastic GPE (SGPEtask/solve_split.m:55-72): the noise of the next step is added
+  // in real space before the forward x transform
+  S_SOPEN = 17,   // block-open half phase, + noise, forward
+  S_SMID = 18,    // inverse, full phase, + noise, forward
+  S_KINM = 19,    // S_KIN with the energy cut-off mask kk < ecut
+  S_NSEQ = 20
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -81,6 +86,12 @@ struct SweepParams {
   const double* k1ax;        // S_KEX/S_KE/S_RCLOSEM: k_i / n of the swept axis
   const double* rcoord;      // coordinate per row (y for x pencils, x for y pencils)
   double rot_w;              // kinetic-sum cross term weight: sum (k^2/2 + rot_w coord k) |F psi|^2 / n
+  // stochastic GPE
+  double noise_var;          // sqrt(T gamma ddt / weight)
+  unsigned long long noise_step, noise_seed;   // Philox counter word 1-2 and key
+  const double* kkcol;       // S_KINM: kx^2 (+ ky^2) per column of the mid axis
+  const double* kkax;        // S_KINM: k^2 per index of the mid axis
+  double ecut;
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

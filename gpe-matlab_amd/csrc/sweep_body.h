@@ -8,6 +8,7 @@
 #pragma once
 #include "fft_dev.h"
 #include "phase_dev.h"
+#include "philox_dev.h"
 #include "reduce_dev.h"
 #include "sweep.h"
 
@@ -16,7 +17,7 @@ namespace gpe {
 template <int SEQ> struct SeqTraits {
   static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM ||
                                  SEQ == S_KEX || SEQ == S_ITPB || SEQ == S_ROPEN || SEQ == S_RMID ||
-                                 SEQ == S_RCLOSEM || SEQ == S_RITPB;
+                                 SEQ == S_RCLOSEM || SEQ == S_RITPB || SEQ == S_SOPEN || SEQ == S_SMID;
   static constexpr bool ROT_KX = SEQ == S_ROPEN || SEQ == S_RMID || SEQ == S_RCLOSEM || SEQ == S_RITPA ||
                                  SEQ == S_RITPB || SEQ == S_RKIN;
   static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE);
@@ -55,7 +56,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   const int c = COLFAST ? tid % C : tid / T;
   const int t = COLFAST ? tid / C : tid % T;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
-  if constexpr (SEQ == S_KIN) {
+  if constexpr (SEQ == S_KIN || SEQ == S_KINM) {
     for (int i = tid; i < N; i += 256) twl[N + i] = p.kax[i];
     __syncthreads();  // single-pass plans have no exchange barrier before the first use
   }
@@ -158,6 +159,33 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
       const double nl = exp_cw(hdt * (n * p.g * mu2 + vv[m]));     // groundstate_itp.m:76 (36: mu = 1)
       if (active) p.nop[base + (unsigned)m * step] = nl;
       v[m] = cscale(cscale(v[m], mu), nl);                         // groundstate_itp.m:52 then 40
+    }
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_KINM) {
+    // ekk .* mask with mask = kk < ecut, kk = (kx^2 + ky^2 + kz^2)/2 summed as the grid does
+    // (SGPEtask/solve_split.m:39-41)
+    const double2 kc = p.kcol[cg];
+    const double kcsum = p.kkcol[cg];
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double kk = (kcsum + p.kkax[t + T * m]) / 2;
+      v[m] = kk < p.ecut ? cmul(cmul(kc, twl[N + t + T * m]), v[m]) : make_double2(0.0, 0.0);
+    }
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_SOPEN || SEQ == S_SMID) {
+    // SGPEtask/solve_split.m:55 (open) or 69 (previous step's phase), then 59 (noise)
+    if constexpr (SEQ == S_SMID) fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+    const double mu = p.st->mu;
+    const double f = SEQ == S_SOPEN ? 0.5 : 1.0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const double a = -(vv[m] - mu) - p.g * cabs2(v[m]);
+      double2 x = mul_cexp(v[m], a * p.dr * f, a * p.di * f, has_re);
+      const double2 z = complex_normal(base + (unsigned)m * step, p.noise_step, p.noise_seed);
+      x.x = x.x + p.noise_var * z.x;
+      x.y = x.y + p.noise_var * z.y;
+      v[m] = x;
     }
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_RKIN) {
@@ -269,7 +297,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                  : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE || SEQ == S_CLOSEM) ? GPE_WAVES_PHASE : 1)))
     sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
+  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM ? 2 * N : N];
   __shared__ double red[4];
   const unsigned tile = p.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
   sweep_body<N, C, COLFAST, SEQ>(p, tile, tile, lds, twl, red);

@@ -28,13 +28,13 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, const SweepParam
       GPE_L(true, S_FWD) GPE_L(true, S_INV) GPE_L(true, S_KIN) GPE_L(true, S_OPEN) GPE_L(true, S_MID)
       GPE_L(true, S_CLOSE) GPE_L(true, S_KEX) GPE_L(true, S_KE) GPE_L(true, S_ITPA) GPE_L(true, S_ITPB)
       GPE_L(true, S_CLOSEM) GPE_L(true, S_ROPEN) GPE_L(true, S_RMID) GPE_L(true, S_RCLOSEM)
-      GPE_L(true, S_RITPA) GPE_L(true, S_RITPB)
+      GPE_L(true, S_RITPA) GPE_L(true, S_RITPB) GPE_L(true, S_SOPEN) GPE_L(true, S_SMID) GPE_L(true, S_KINM)
       default: return hipErrorInvalidValue;
     }
   } else {
     switch (seq) {
       GPE_L(false, S_FWD) GPE_L(false, S_INV) GPE_L(false, S_KIN) GPE_L(false, S_KE)
-      GPE_L(false, S_KEX) GPE_L(false, S_RKIN)
+      GPE_L(false, S_KEX) GPE_L(false, S_RKIN) GPE_L(false, S_KINM)
       default: return hipErrorInvalidValue;
     }
   }

@@ -44,6 +44,11 @@ class Itp2Args(C.Structure):
                 ("M2", C.c_double)]
 
 
+class SgpeArgs(C.Structure):
+    _fields_ = [("ddt", C.c_double), ("niter_inner", C.c_int), ("T", C.c_double), ("ecut", C.c_double),
+                ("seed", C.c_ulonglong)]
+
+
 class ItpArgs(C.Structure):
     _fields_ = [("dt", C.c_double), ("eps", C.c_double), ("max_iter", C.c_int), ("min_iter", C.c_int),
                 ("adjust_stepsize", C.c_int), ("nargout", C.c_int)]
@@ -63,6 +68,9 @@ SIGNATURES = {
     "gpe_get_state": [_P, C.c_int, _DP],
     "gpe_set_coupling": [_P, _DP, C.c_int],
     "gpe_set_omega": [_P, _D],
+    "gpe_sgpe_begin": [_P, C.POINTER(SgpeArgs)],
+    "gpe_sgpe_block": [_P, C.c_longlong, _DP],
+    "gpe_sgpe_outer": [_P, C.c_int, _DP, _DP],
     "gpe_vtd_set": [_P, C.c_int, _DP, _DP],
     "gpe_vtd_coeffs": [_P, _DP, C.c_int],
     "gpe_set_potential_mu": [_P, C.c_int, _DP],
@@ -233,6 +241,20 @@ class Device:
             return
         v = to_colmajor_f64(np.broadcast_to(V, self.shape))
         self.call("gpe_set_potential_mu", comp, dptr(v))
+
+    def sgpe_begin(self, ddt, niter_inner, T, ecut, seed):
+        a = SgpeArgs(float(ddt), int(niter_inner), float(T), float(ecut), int(seed) & 0xFFFFFFFFFFFFFFFF)
+        self.call("gpe_sgpe_begin", C.byref(a))
+
+    def sgpe_block(self, step0):
+        n = C.c_double()
+        self.call("gpe_sgpe_block", int(step0), C.byref(n))
+        return n.value
+
+    def sgpe_outer(self, j):
+        t2, n = C.c_double(), C.c_double()
+        self.call("gpe_sgpe_outer", int(j), C.byref(t2), C.byref(n))
+        return t2.value, n.value
 
     def set_omega(self, omega):
         self.call("gpe_set_omega", float(omega))

@@ -289,6 +289,60 @@ class GPEtask:
         return out[0] if nargout == 1 else tuple(out)
 
 
+class SGPEtask(GPEtask):
+    """Drop-in for lib/@SGPEtask (SGPEtask.m:1-47): the stochastic GPE at temperature T
+    with an optional energy cut-off ecut (omega = 0).
+
+    Deviation: MATLAB's randn stream is replaced by a counter-based Philox4x32-10
+    stream keyed by ``noise_seed`` (one complex normal variate per grid point and
+    global step), so runs are reproducible and resumable; the oracle draws the same
+    variates."""
+
+    def __init__(self, grid, trappot, seed=None):
+        super().__init__(grid, trappot, seed)
+        self.T = 0.0                 # SGPEtask.m:5
+        self.snapshots = []          # SGPEtask.m:6
+        self.ss_maxlength = 100      # SGPEtask.m:7
+        self.ecut = 0.0              # SGPEtask.m:8
+        self.noise_seed = 0
+
+    def _check_omega(self):
+        if not np.isscalar(self.omega) or self.omega != 0:
+            raise NotImplementedError("the stochastic GPE runs without rotation (omega = 0)")
+
+    def solve_split(self, ddt, niter_inner, niter_outer):
+        """SGPEtask/solve_split.m:1-81 on the GPU; returns the final state."""
+        self._check_omega()
+        dev = self._device()
+        start = self.current_iter
+        self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+        if start > 0:                                           # 31-35
+            if not self._dev_state_valid:
+                self._upload(self.current_state)
+        else:
+            self._upload(self.init_state)
+        self._dev_state_valid = True
+        static = self._static_potential()
+        dev.set_potential(0, self.Vtrap if static else self.getVtotal(self.current_time))
+        dev.sgpe_begin(ddt, niter_inner, self.T, self.ecut, self.noise_seed)
+        n_rec = self.n_recalc
+        dt_outer = ddt * niter_inner
+        mu = self.mu_init                                       # 50
+        for j in range(start + 1, niter_outer + 1):             # 49
+            if static:
+                time2, ncur = dev.sgpe_outer(j)
+            else:
+                time = (j - 1) * dt_outer
+                for jj in range(1, niter_inner // n_rec + 1):   # 52
+                    time2 = time + (jj - 1) * ddt * n_rec       # 53
+                    dev.set_potential(0, self.getVtotal(time2))   # 54
+                    ncur = dev.sgpe_block((j - 1) * niter_inner + (jj - 1) * n_rec)
+            err = self.ext_callback(j, time2, mu, ncur)         # 77-80
+            if err == "TERM":
+                break
+        return self.current_state
+
+
 class GPEtaskSpinor(GPEtask):
     """Drop-in for lib/@GPEtaskSpinor (GPEtaskSpinor.m:1-146): two coherently coupled
     components with a scalar or space-dependent complex Rabi coupling.

@@ -161,6 +161,28 @@ typedef struct {
 int gpe_itp2(gpe_handle* h, const gpe_itp2_args* a, double* MU, double* MU2, double* EE, int* n_iter,
              double* dt_final, int* converged);
 
+/* Stochastic GPE: SGPEtask.solve_split (lib/@SGPEtask/solve_split.m:1-81), one
+ * component, omega = 0, from the uploaded state and potential.  begin sets the
+ * step (dt = ddt i/(1+i gamma)), the kinetic factor (masked to kk < ecut when
+ * ecut > 0 and T > 0), the noise amplitude sqrt(T gamma ddt / weight) and mu =
+ * mu_init.  The noise of global step s at grid point p (column-major index) is one
+ * complex normal variate from Philox4x32-10 with counter (p, s_lo, s_hi, 0) and key
+ * (seed_lo, seed_hi), Box-Muller on two 53-bit uniforms (no reference counterpart:
+ * MATLAB's randn stream).  gpe_sgpe_block runs one block whose first step is global
+ * step step0 (a time-dependent potential is uploaded before it); gpe_sgpe_outer runs
+ * outer iteration j with steps (j-1)*niter_inner + ...  ncur = integrate(|psi|^2)
+ * after the block (solve_split.m:75). */
+typedef struct {
+  double ddt;
+  int niter_inner;
+  double T;                 /* SGPEtask.m:5 */
+  double ecut;              /* SGPEtask.m:8 */
+  unsigned long long seed;  /* noise key */
+} gpe_sgpe_args;
+int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a);
+int gpe_sgpe_block(gpe_handle* h, long long step0, double* ncur);
+int gpe_sgpe_outer(gpe_handle* h, int j, double* time2, double* ncur);
+
 /* ------------------------------------------------------------------ adjoint
  * (no reference counterpart: north-star form 3, the gradient entry point that
  * plays the torch.autograd.Function.backward role).

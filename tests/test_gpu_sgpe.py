@@ -1,0 +1,66 @@
+"""GPU parity of the stochastic GPE (SGPEtask.solve_split) against the oracle, which
+draws the same counter-based noise (oracle/philox.py)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(dims, n, xmax, T=0.5, ecut=0.0, g=0.4, gamma=0.05, mu=1.5, seed=21):
+    import gpe_hip
+    if dims == 2:
+        G, O = gpe_hip.grid2d(xmax[0], n[0], xmax[1], n[1]), oracle.grid2d(xmax[0], n[0], xmax[1], n[1])
+    else:
+        G = gpe_hip.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2])
+        O = oracle.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2])
+    r2 = G.mesh.x ** 2 + G.mesh.y ** 2 + (G.mesh.z ** 2 if dims == 3 else 0)
+    V = 0.5 * r2
+    psi = np.exp(-r2 / 3) * np.exp(0.3j * G.mesh.x)
+    tg, to = gpe_hip.SGPEtask(G, V), oracle.SGPEtask(O, V)
+    for t in (tg, to):
+        t.g, t.gamma, t.T, t.ecut, t.mu_init, t.noise_seed = g, gamma, T, ecut, mu, seed
+        t.init_state = psi.copy()
+    return tg, to
+
+
+CASES = [(2, (32, 24), (6, 5)), (2, (64, 64), (8, 8)), (3, (16, 24, 12), (5, 6, 4)), (3, (32, 32, 16), (6, 6, 4))]
+
+
+@pytest.mark.parametrize("dims,n,xmax", CASES)
+@pytest.mark.parametrize("ecut", [0.0, 6.0])
+def test_sgpe_parity(dims, n, xmax, ecut):
+    tg, to = _pair(dims, n, xmax, ecut=ecut)
+    pg, po = tg.solve_split(0.005, 20, 3), to.solve_split(0.005, 20, 3)
+    err = rel_l2(pg, po)
+    print(f"SGPE {dims}D {n} ecut={ecut}: rel-L2 {err:.2e}")
+    assert err < 1e-11
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
+    assert np.all(tg.history["mu"] == 1.5)
+
+
+def test_sgpe_resume_is_the_same_run():
+    ta, _ = _pair(2, (32, 32), (6, 6))
+    tb, _ = _pair(2, (32, 32), (6, 6))
+    pa = ta.solve_split(0.005, 20, 4)
+    tb.solve_split(0.005, 20, 2)
+    pb = tb.solve_split(0.005, 20, 4)
+    assert np.array_equal(pa, pb)
+
+
+def test_sgpe_zero_temperature_is_damped_gpe_with_fixed_mu():
+    import gpe_hip
+    ts, _ = _pair(2, (32, 32), (6, 6), T=0.0)
+    tg = gpe_hip.GPEtask(ts.grid, ts.Vtrap)
+    tg.g, tg.gamma, tg.mu_init = ts.g, ts.gamma, ts.mu_init
+    tg.init_state = ts.init_state.copy()
+    assert rel_l2(ts.solve_split(0.005, 20, 3), tg.solve_split(0.005, 20, 3)) < 1e-13
+
+
+def test_sgpe_callback_term_stops():
+    ts, _ = _pair(2, (32, 24), (6, 5))
+    ts.user_callback = lambda task: "TERM" if task.current_iter >= 2 else ""
+    ts.solve_split(0.005, 20, 5)
+    assert ts.current_iter == 2
