@@ -36,7 +36,7 @@ PEAK_HBM_GBPS = 8000.0
 # algorithmic HBM bytes per grid point per launch of each sweep sequence
 SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40, "kex": 24, "ke": 16,
              "itpa": 40, "itpb": 48, "closem": 40, "rkin": 32, "ropen": 40, "rmid": 40, "rclosem": 40,
-             "ritpa": 40, "ritpb": 48}
+             "ritpa": 40, "ritpb": 48, "sopen": 40, "smid": 40, "kinm": 32}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 
 

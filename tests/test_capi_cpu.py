@@ -71,3 +71,13 @@ def test_package_fails_loudly_without_library(monkeypatch, tmp_path):
     with pytest.raises(L.GPELibraryError):
         L.load()
     importlib.reload(L)
+
+
+def test_profile_names_cover_every_sweep_sequence():
+    # gpe_profile_read rejects a buffer shorter than 3 * S_NSEQ entries
+    src = open(os.path.join(ROOT, "gpe-matlab_amd", "csrc", "sweep.h")).read()
+    nseq = int(re.search(r"S_NSEQ\s*=\s*(\d+)", src).group(1))
+    from gpe_hip import _lib
+    assert sorted(_lib.SEQ_NAMES) == list(range(nseq))
+    import bench
+    assert set(_lib.SEQ_NAMES.values()) <= set(bench.SEQ_BYTES)
