@@ -170,6 +170,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->sg_kkcol);
   (void)hipFree(h->sg_kkax);
   (void)hipFree(h->vtd_G);
+  (void)hipFree(h->dc_buf);
   (void)hipFree(h->vtd_V0);
   (void)hipFree(h->itp2_P);
   for (double* b : h->itp2_hist) (void)hipFree(b);

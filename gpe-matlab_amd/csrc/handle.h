@@ -116,6 +116,9 @@ struct gpe_handle {
   double* rb_out = nullptr;       // 2 reduced gradient components
   double2* kcol_rc[3] = {nullptr, nullptr, nullptr};   // conj(exp(-kk dt/2)), conj(exp(-kk dt)),
   double2* kax_rc[3] = {nullptr, nullptr, nullptr};    // conj(exp(-kk dt) exp(kk dt/2))
+  // vortex-core detection (gpe_detect_core*): one grown scratch allocation
+  char* dc_buf = nullptr;
+  size_t dc_bytes = 0;
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)

@@ -71,6 +71,8 @@ SIGNATURES = {
     "gpe_sgpe_begin": [_P, C.POINTER(SgpeArgs)],
     "gpe_sgpe_block": [_P, C.c_longlong, _DP],
     "gpe_sgpe_outer": [_P, C.c_int, _DP, _DP],
+    "gpe_detect_core": [_P, C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_int, _DP, _IP, _DP, _IP],
+    "gpe_detect_core_array": [_P, _DP, C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_int, _DP, _IP, _DP, _IP],
     "gpe_vtd_set": [_P, C.c_int, _DP, _DP],
     "gpe_vtd_coeffs": [_P, _DP, C.c_int],
     "gpe_set_potential_mu": [_P, C.c_int, _DP],
@@ -258,6 +260,25 @@ class Device:
 
     def set_omega(self, omega):
         self.call("gpe_set_omega", float(omega))
+
+    def detect_core(self, x, y, refine=0, comp=0, zslice=0, phi=None, cap=64):
+        """gpe_detect_core on the resident state (or gpe_detect_core_array on ``phi``,
+        a [ny, nx] host array); returns (resp, resm) as (n, 2) arrays of [x, y]."""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+        v = None if phi is None else to_colmajor_c128(phi)
+        while True:
+            rp, rm = np.zeros((cap, 2)), np.zeros((cap, 2))
+            n_p, n_m = C.c_int(), C.c_int()
+            if v is None:
+                self.call("gpe_detect_core", int(comp), int(zslice), dptr(x), dptr(y), int(refine), int(cap),
+                          dptr(rp), C.byref(n_p), dptr(rm), C.byref(n_m))
+            else:
+                self.call("gpe_detect_core_array", v.view(np.float64).ctypes.data_as(_DP), int(y.size), int(x.size),
+                          dptr(x), dptr(y), int(refine), int(cap), dptr(rp), C.byref(n_p), dptr(rm), C.byref(n_m))
+            if n_p.value <= cap and n_m.value <= cap:
+                return rp[:n_p.value].copy(), rm[:n_m.value].copy()
+            cap = max(n_p.value, n_m.value)
 
     def measure(self):
         out = np.zeros(5)

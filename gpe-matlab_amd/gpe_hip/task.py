@@ -60,6 +60,7 @@ class GPEtask:
         self._host_state = None      # host copy of current_state
         self._dev_state_valid = False  # device holds current_state (maybe newer than host copy)
         self._host_stale = False
+        self._in_callback = False    # inside user_callback: the solver's state, V and params are resident
 
     # ------------------------------------------------------------------ state
     def _device(self):
@@ -126,9 +127,20 @@ class GPEtask:
         self._hist_set("n", step, n)
         res_text = ""
         if callable(self.user_callback):
-            r = self.user_callback(self)
+            self._in_callback = True
+            try:
+                r = self.user_callback(self)
+            finally:
+                self._in_callback = False
             res_text = r if isinstance(r, str) else ""
         return res_text
+
+    def _resident(self, phi, time):
+        """True inside user_callback for the current state at current_time: the device then
+        holds that state, the potential of current_time (the last block's, uploaded or built
+        at its start time2 = current_time) and the run's parameters, so reductions need no
+        upload and leave the run untouched."""
+        return phi is None and self._in_callback and (time is None or time == self.current_time)
 
     # ------------------------------------------------------------------ energies
     def get_energy(self, phi=None, time=None):
@@ -141,9 +153,35 @@ class GPEtask:
         m = self._measure(phi, time)
         return (m[1] + m[3]) + self.g * m[2]
 
+    # ------------------------------------------------------------------ diagnostics
+    def detect_core(self, refine=0, zslice=None, comp=0):
+        """lib/util/detect_core.m:1-66 on the resident state, without copying it to the host:
+        detect_core(phi(:,:,zslice+1), mesh.x(:,:,zslice+1), mesh.y(:,:,zslice+1), refine)
+        with phi = current_state (component ``comp`` of a two-component task).  zslice is
+        0-based and required on a 3D grid.  Returns (resp, resm), (n, 2) arrays of [x, y]."""
+        dev = self._device()
+        mesh = self.grid.mesh
+        if self.grid.ndims == 3:
+            if zslice is None:
+                raise ValueError("detect_core on a 3D grid needs a z slice")
+            x, y = mesh.x[0, :, 0], mesh.y[:, 0, 0]
+        else:
+            if zslice not in (None, 0):
+                raise ValueError("a 2D grid has only z slice 0")
+            zslice = 0
+            x, y = mesh.x[0, :], mesh.y[:, 0]
+        if not self._dev_state_valid:
+            if self.current_state is None:
+                raise ValueError("no current state")
+            self._upload(self.current_state)
+            self._dev_state_valid = True
+        return dev.detect_core(x, y, refine, comp, zslice)
+
     def _measure(self, phi, time):
         self._check_omega()
         dev = self._device()
+        if self._resident(phi, time):
+            return dev.measure()
         self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
         if time is None:
             time = self.current_time
@@ -409,6 +447,9 @@ class GPEtaskSpinor(GPEtask):
     def _measure_sp(self, phi, time):
         self._check_omega()
         dev = self._device()
+        if self._resident(phi, time):
+            dev.set_potential_mu(0, None)   # the step potential at current_time; the next block-end
+            return dev.measure()            # potential is set again before it is used
         self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
         if time is None:
             time = self.current_time

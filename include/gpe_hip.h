@@ -183,6 +183,24 @@ int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a);
 int gpe_sgpe_block(gpe_handle* h, long long step0, double* ncur);
 int gpe_sgpe_outer(gpe_handle* h, int j, double* time2, double* ncur);
 
+/* ------------------------------------------------------------------ diagnostics
+ * Vortex cores of a 2D slice, lib/util/detect_core.m:1-66 called as
+ * detect_core(phi(:,:,zslice+1), mesh.x(:,:,zslice+1), mesh.y(:,:,zslice+1), refine),
+ * computed on the device so that a user_callback (GPEtask.m:128-135) needs no copy of
+ * the state.  x[nx] = rx(1,:) and y[ny] = ry(:,1) are the coordinates of the columns
+ * and rows.  resp / resm receive up to `cap` rows [x, y] (row-major) of the positively
+ * / negatively charged cores in MATLAB's column-major find order; *np / *nm are the
+ * numbers found (rows beyond cap are not written).  With refine != 0 each position is
+ * the zero of the bilinear interpolants of Re and Im on the plaquette
+ * (detect_core.m:54-64), [0, 0] when none lies inside.  Needs even nx, ny >= 4.
+ * gpe_detect_core reads component comp of the resident state and leaves the solver
+ * state untouched; gpe_detect_core_array takes a host array phi[2*ny*nx] (interleaved
+ * complex, column-major) and uses h only for its device, stream and error slot. */
+int gpe_detect_core(gpe_handle* h, int comp, int zslice, const double* x, const double* y, int refine, int cap,
+                    double* resp, int* np, double* resm, int* nm);
+int gpe_detect_core_array(gpe_handle* h, const double* phi, int ny, int nx, const double* x, const double* y,
+                          int refine, int cap, double* resp, int* np, double* resm, int* nm);
+
 /* ------------------------------------------------------------------ adjoint
  * (no reference counterpart: north-star form 3, the gradient entry point that
  * plays the torch.autograd.Function.backward role).

@@ -19,3 +19,4 @@ those checks.
 """
 from .grids import Grid2D, Grid3D, grid2d, grid3d  # noqa: F401
 from .gpetask import GPEtask, GPEtask2comp, GPEtaskSpinor, SGPEtask  # noqa: F401
+from .diagnostics import detect_core  # noqa: F401
