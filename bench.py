@@ -78,7 +78,7 @@ def main():
                     help="c2: rotation frequency (rotating-frame kinetic step); reported as a variant")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (known-size kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2", "vtd", "sgpe"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "itp2", "vtd", "sgpe", "diag"],
                     help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
                          "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
@@ -106,7 +106,7 @@ def main():
         sys.exit(2)
     if args.workload != "c2":
         {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "itp2": bench_itp2,
-         "vtd": bench_vtd, "sgpe": bench_sgpe}[args.workload](args)
+         "vtd": bench_vtd, "sgpe": bench_sgpe, "diag": bench_diag}[args.workload](args)
         return
 
     import gpe_hip
@@ -668,6 +668,100 @@ def bench_sgpe(args):
                                  "ecut=60, mu_init=75, dt=0.002, n_recalc=10", "grid": [args.n] * 3},
           step_alg_GBps=round(gbps, 1), step_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
           parity_rel_l2_vs_oracle_10_steps=parity)
+
+
+def bench_diag(args):
+    """Diagnostics in user_callback after every outer step of 50 split steps, as the
+    post_process of examples/example2 (detect_core(phi, X, Y, 1) plus get_energy):
+    on the device (GPEtask.detect_core / get_energy on the resident state) against a host
+    copy of the state with detect_core on the CPU.  3D: the 256^3 workload with the
+    middle z slice; 2D: 256^2 with three imprinted vortices.  Rates are differenced
+    between runs of 1 and 1 + n outer steps, so set-up and uploads cancel."""
+    import gpe_hip
+    import oracle
+    from gpe_hip import workloads
+    nout = max(4, 2 * args.steps // 50)
+    out = {}
+    for dims in (3, 2):
+        if dims == 3:
+            grid, V, psi0, props = workloads.c2_state(n=args.n)
+            k = args.n // 2
+            X2, Y2 = grid.mesh.x[:, :, k], grid.mesh.y[:, :, k]
+            ddt = 0.002
+        else:
+            grid = gpe_hip.grid2d(17, 256, 17, 256)
+            X2, Y2 = grid.mesh.x, grid.mesh.y
+            V = 0.5 * (X2 ** 2 + Y2 ** 2)
+            props = dict(Ntotal=1e5, g=0.06, gamma=0.01)
+            psi0, _ = workloads.thomas_fermi(grid, V, props["Ntotal"], props["g"])
+            for x0, y0, sg in ((5.0, 0.0, 1), (0.0, 5.0, -1), (-4.0, -3.0, 1)):
+                psi0 = psi0 * np.tanh(np.hypot(X2 - x0, Y2 - y0) / 0.4) * np.exp(1j * sg * np.arctan2(Y2 - y0, X2 - x0))
+            k = 0
+            ddt = 0.004
+        t = gpe_hip.GPEtask(grid, V)
+        t.Ntotal, t.g, t.gamma = props["Ntotal"], props["g"], props["gamma"]
+        rec = []
+
+        cost = {"device": [], "host": []}
+
+        def cb_device(task):
+            c0 = time.perf_counter()
+            rec.append((task.detect_core(refine=1, zslice=k if dims == 3 else None), task.get_energy()))
+            cost["device"].append(time.perf_counter() - c0)   # both calls return host values: synchronous
+
+        def cb_host(task):
+            c0 = time.perf_counter()
+            phi = task.current_state                       # the whole state to the host
+            rec.append(oracle.detect_core(phi[:, :, k] if dims == 3 else phi, X2, Y2, 1))
+            cost["host"].append(time.perf_counter() - c0)
+
+        el = {}
+        cores_dev = cores_host = None
+        for mode, cb in (("warm", None), ("none", None), ("device", cb_device), ("host", cb_host)):
+            # differenced: a run of 1 outer step and one of 1 + nout (same set-up and uploads)
+            tt = []
+            for n_outer in ((1,) if mode == "warm" else (1, 1 + nout)):
+                t.init_state = psi0
+                t.current_iter = 0
+                t.current_time = 0.0
+                t.user_callback = cb
+                rec.clear()
+                sync()
+                t0 = time.perf_counter()
+                t.solve_split(ddt, 50, n_outer)
+                sync()
+                tt.append(time.perf_counter() - t0)
+            el[mode] = tt[-1] - tt[0] if len(tt) == 2 else tt[0]
+            if mode == "device":
+                cores_dev = rec[-1][0]
+            if mode == "host":
+                cores_host = rec[-1]
+        same = all(a.shape == b.shape and np.allclose(a, b, rtol=0, atol=1e-10) for a, b in zip(cores_dev, cores_host))
+        steps = 50 * nout
+        r = {m: steps / el[m] for m in ("none", "device", "host")}
+        ov = {m: 1e3 * float(np.median(cost[m])) for m in ("device", "host")}   # per callback
+        c0 = time.perf_counter()
+        oracle.detect_core(np.asarray(t.current_state[:, :, k] if dims == 3 else t.current_state), X2, Y2, 1)
+        cpu_ms = 1e3 * (time.perf_counter() - c0)
+        log(f"diag {dims}D: {steps} steps; steps/s no callback {r['none']:.1f}, device diagnostics {r['device']:.1f} "
+            f"({ov['device']:.3f} ms per callback, median), host copy + CPU detect_core "
+            f"{r['host']:.1f} ({ov['host']:.2f} ms per callback); cores {cores_dev[0].shape[0]}+/{cores_dev[1].shape[0]}- "
+            f"same as host: {same}; CPU detect_core alone {cpu_ms:.1f} ms")
+        out[dims] = dict(r=r, ov=ov, same=same, cpu_ms=cpu_ms, steps=steps, el=el)
+    d3, d2 = out[3], out[2]
+    _line("split-steps/s with detect_core(refine) + get_energy after every 50 steps (256^3)",
+          round(d3["r"]["device"], 2), "split-steps/s", d3["steps"], 50, d3["el"]["device"], True,
+          {"workload": "GPEtask 3D 256^3 (as the default workload), solve_split(0.002, 50, n) with a user_callback: "
+                       "detect_core on z slice 128 with refine + get_energy", "grid": [args.n] * 3},
+          no_callback_steps_per_s=round(d3["r"]["none"], 2), device_callback_ms=round(d3["ov"]["device"], 3),
+          host_copy_callback_steps_per_s=round(d3["r"]["host"], 2), host_copy_callback_ms=round(d3["ov"]["host"], 3),
+          cores_match_host=d3["same"],
+          cpu_baseline={"value": round(d3["cpu_ms"], 2), "unit": "ms per detect_core(refine=1) on one 256^2 slice",
+                        "cores": 1, "kind": "port", "sample": "oracle detect_core on the final middle slice"},
+          d2_256=dict(no_callback_steps_per_s=round(d2["r"]["none"], 1), device_steps_per_s=round(d2["r"]["device"], 1),
+                      device_callback_ms=round(d2["ov"]["device"], 3), host_steps_per_s=round(d2["r"]["host"], 1),
+                      host_callback_ms=round(d2["ov"]["host"], 3), cores_match_host=d2["same"],
+                      cpu_detect_core_ms=round(d2["cpu_ms"], 2)))
 
 
 if __name__ == "__main__":
