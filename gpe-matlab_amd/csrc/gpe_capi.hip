@@ -167,8 +167,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->kcol_c);
   (void)hipFree(h->kax_c);
   for (double* b : h->Vmu) (void)hipFree(b);
-  (void)hipFree(h->sg_kkcol);
-  (void)hipFree(h->sg_kkax);
+  (void)hipFree(h->sg_kmax);
   (void)hipFree(h->vtd_G);
   (void)hipFree(h->dc_buf);
   (void)hipFree(h->vtd_V0);

@@ -88,10 +88,9 @@ struct gpe_handle {
   double2* kax_c = nullptr;
   // stochastic GPE (gpe_sgpe_*)
   bool sg_begun = false, sg_mask = false;
-  double sg_var = 0, sg_ecut = 0;
+  double sg_var = 0;
   unsigned long long sg_seed = 0;
-  double* sg_kkcol = nullptr;     // kx^2 (+ ky^2) per mid-axis column
-  double* sg_kkax = nullptr;      // k^2 per mid-axis index
+  int* sg_kmax = nullptr;         // per mid-axis column: largest |k| index with kk < ecut
   // two-component block-end reductions: potential at task.current_time (gpe_set_potential_mu)
   double* Vmu[2] = {nullptr, nullptr};
   bool use_vmu = false;

@@ -20,9 +20,7 @@ SweepParams sg_params(gpe_handle* h, int axis) {
   SweepParams p = base_params(h, axis, h->psi[0], h->V[0]);
   p.noise_var = h->sg_var;
   p.noise_seed = h->sg_seed;
-  p.kkcol = h->sg_kkcol;
-  p.kkax = h->sg_kkax;
-  p.ecut = h->sg_ecut;
+  p.kmax = h->sg_kmax;
   return p;
 }
 
@@ -77,7 +75,6 @@ int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a) {
   h->di = dt.imag();
   if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;                 // 41 / 43
   h->sg_mask = a->ecut > 0 && a->T > 0;                                         // 39
-  h->sg_ecut = a->ecut;
   h->sg_var = std::sqrt(a->T * h->prm.gamma * a->ddt / h->w);                 // 47
   h->sg_seed = a->seed;
   if (h->sg_mask) {
@@ -92,10 +89,18 @@ int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a) {
       for (int ix = 0; ix < h->nx; ++ix) col[ix] = h->ax[0].k[ix] * h->ax[0].k[ix];
     }
     for (int i = 0; i < m.n; ++i) ax[i] = m.k[i] * m.k[i];
-    if (!h->sg_kkcol) HIPCHK(h, hipMalloc(&h->sg_kkcol, col.size() * sizeof(double)));
-    if (!h->sg_kkax) HIPCHK(h, hipMalloc(&h->sg_kkax, ax.size() * sizeof(double)));
-    HIPCHK(h, hipMemcpy(h->sg_kkcol, col.data(), col.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->sg_kkax, ax.data(), ax.size() * sizeof(double), hipMemcpyHostToDevice));
+    // k_{n-a} = -k_a, and k_a^2 grows with a up to the Nyquist index n/2, so the mask of a
+    // column is |k| <= kmax: the longest prefix a = 0, 1, ... with kk < ecut
+    for (int q = 1; q < m.n - q; ++q)
+      if (ax[q] != ax[m.n - q]) return fail(h, GPE_EINVAL, "asymmetric k table on the mid axis");
+    std::vector<int> kmax((size_t)m.ncols);
+    for (size_t c = 0; c < col.size(); ++c) {
+      int q = -1;
+      while (q < m.n / 2 && (col[c] + ax[q + 1]) / 2 < a->ecut) ++q;
+      kmax[c] = q;
+    }
+    if (!h->sg_kmax) HIPCHK(h, hipMalloc(&h->sg_kmax, kmax.size() * sizeof(int)));
+    HIPCHK(h, hipMemcpy(h->sg_kmax, kmax.data(), kmax.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   DevState s;
   if ((rc = read_state(h, &s))) return rc;

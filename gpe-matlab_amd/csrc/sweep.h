@@ -89,9 +89,7 @@ struct SweepParams {
   // stochastic GPE
   double noise_var;          // sqrt(T gamma ddt / weight)
   unsigned long long noise_step, noise_seed;   // Philox counter word 1-2 and key
-  const double* kkcol;       // S_KINM: kx^2 (+ ky^2) per column of the mid axis
-  const double* kkax;        // S_KINM: k^2 per index of the mid axis
-  double ecut;
+  const int* kmax;           // S_KINM: largest |k| index of the mid axis kept in each column (-1: none)
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

@@ -162,15 +162,20 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
     }
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_KINM) {
-    // ekk .* mask with mask = kk < ecut, kk = (kx^2 + ky^2 + kz^2)/2 summed as the grid does
-    // (SGPEtask/solve_split.m:39-41)
+    // ekk .* mask with mask = kk < ecut (SGPEtask/solve_split.m:39-41).  Along the axis kk
+    // grows with |k|, so a column keeps the indices with |k| <= kmax[column] (the host
+    // evaluates kk = (kx^2 + ky^2 + kz^2)/2 as the grid sums it to find kmax).
     const double2 kc = p.kcol[cg];
-    const double kcsum = p.kkcol[cg];
+    const int km = p.kmax[cg];
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const double kk = (kcsum + p.kkax[t + T * m]) / 2;
-      v[m] = kk < p.ecut ? cmul(cmul(kc, twl[N + t + T * m]), v[m]) : make_double2(0.0, 0.0);
+      const int j = t + T * m;
+      const int a = j <= N / 2 ? j : N - j;
+      // the factor times the 0/1 mask, as ekk .* mask (a select here costs registers)
+      const double keep = a <= km ? 1.0 : 0.0;
+      const double2 f = cmul(kc, twl[N + j]);
+      v[m] = cmul(make_double2(f.x * keep, f.y * keep), v[m]);
     }
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_SOPEN || SEQ == S_SMID) {
