@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) adj_x_kernel(const AdjParams p) {
   const int t = tid / C;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
 
-  const unsigned cgu = blockIdx.x * (unsigned)C + (unsigned)c;
+  const unsigned cgu = (C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x) * (unsigned)C + (unsigned)c;
   const bool active = cgu < p.ncols;
   const unsigned cg = active ? cgu : p.ncols - 1u;
   const unsigned base = (cg % p.A) * p.sa + (cg / p.A) * p.sb + (unsigned)t * p.stride;

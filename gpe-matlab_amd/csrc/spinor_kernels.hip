@@ -98,17 +98,6 @@ enum CplKind : int {
 #define GPE_ADJ_WAVES 2
 #endif
 
-// Work-groups are dealt to the 8 XCDs round robin (group b runs on XCD b % 8).  A tile
-// of C2 = 8 pencils covers 64 bytes of a row of V1 / V2, half an L2 line, so neighbouring
-// tiles 2j and 2j+1 go to groups on the same XCD and its L2 fetches each line once
-// (groups beyond the last multiple of 16 keep their own index).
-__device__ __forceinline__ unsigned xcd_pair_tile(unsigned b, unsigned nb) {
-  const unsigned full = nb & ~15u;
-  if (b >= full) return b;
-  const unsigned x = b & 7u, s = b >> 3;
-  return ((s >> 1) << 4) + (x << 1) + (s & 1u);
-}
-
 template <int N, int C2, int SEQ, int CK>
 __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   using PL = Plan<N>;

@@ -48,6 +48,17 @@ enum Slot : int {
 };
 
 // Scalars that evolve on the device between kernels (no host round trip).
+// Work-groups are dealt to the 8 XCDs round robin (group b runs on XCD b % 8).  When a
+// tile of pencils covers only half of a 128-byte L2 line of an 8-byte array (8 pencils
+// in a row of V), neighbouring tiles 2j and 2j+1 go to groups on the same XCD, so its L2
+// fetches each line once; groups beyond the last multiple of 16 keep their index.
+__device__ __forceinline__ unsigned xcd_pair_tile(unsigned b, unsigned nb) {
+  const unsigned full = nb & ~15u;
+  if (b >= full) return b;
+  const unsigned x = b & 7u, s = b >> 3;
+  return ((s >> 1) << 4) + (x << 1) + (s & 1u);
+}
+
 struct DevState {
   double mu;      // chemical potential in VV = V - mu
   double scale;   // pending real scale of the stored state (renormalisation)

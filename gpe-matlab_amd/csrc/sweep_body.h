@@ -304,7 +304,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM ? 2 * N : N];
   __shared__ double red[4];
-  const unsigned tile = p.reverse ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+  const unsigned b = C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const unsigned tile = p.reverse ? gridDim.x - 1u - b : b;
   sweep_body<N, C, COLFAST, SEQ>(p, tile, tile, lds, twl, red);
 }
 
