@@ -38,10 +38,23 @@ __device__ __forceinline__ void finalize_core(const FinParams& f, double* red) {
     acc[q] = 0.0;
     maxc = f.cnt[slots[q]] > maxc ? f.cnt[slots[q]] : maxc;
   }
-  for (int i = threadIdx.x; i < maxc; i += 256) {
+  // four strides of loads in flight per pass; each thread still adds its partials in
+  // index order (t, t + 256, t + 512, ...)
+  for (int i0 = threadIdx.x; i0 < maxc; i0 += 4 * 256) {
+    double v[4][NS];
 #pragma unroll
-    for (int q = 0; q < NS; ++q)
-      if (i < f.cnt[slots[q]]) acc[q] += f.partials[slots[q] * f.maxwg + i];
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * 256;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) v[u][q] = i < f.cnt[slots[q]] ? f.partials[slots[q] * f.maxwg + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * 256;
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        if (i < f.cnt[slots[q]]) acc[q] += v[u][q];
+    }
   }
   double tot[NS];
 #pragma unroll
