@@ -209,9 +209,14 @@ int gpe_set_params(gpe_handle* h, const gpe_params* p) {
     HIPCHK(h, hipMalloc(&h->V[1], h->npts * sizeof(double)));
     HIPCHK(h, hipMemset(h->psi[1], 0, h->npts * sizeof(double2)));
   }
+  const gpe_params& o = h->prm;
+  const bool same = h->have_params && o.ncomp == p->ncomp && o.g[0] == p->g[0] && o.g[1] == p->g[1] &&
+                    o.g[2] == p->g[2] && o.g[3] == p->g[3] && o.gamma == p->gamma &&
+                    o.decay_rate == p->decay_rate && o.Ntotal == p->Ntotal && o.mu_init == p->mu_init &&
+                    o.n_recalc == p->n_recalc && o.nlincpl == p->nlincpl;
   h->prm = *p;
   h->have_params = true;
-  h->begun = false;
+  if (!same) h->begun = false;   // unchanged parameters keep a begun run going (reductions in callbacks)
   return GPE_OK;
 }
 

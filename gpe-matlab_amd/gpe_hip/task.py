@@ -182,7 +182,8 @@ class GPEtask:
         dev = self._device()
         if self._resident(phi, time):
             return dev.measure()
-        self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+        if not self._in_callback:                # inside a callback the run's parameters stay
+            self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
         if time is None:
             time = self.current_time
         dev.set_potential(0, self.getVtotal(time))
@@ -190,12 +191,15 @@ class GPEtask:
             if not self._dev_state_valid:
                 self._upload(self.current_state)
                 self._dev_state_valid = True
-            return dev.measure()
-        saved = self._fetch() if self._dev_state_valid else None
-        self._upload(phi)
-        m = dev.measure()
-        if saved is not None:
-            self._upload(saved)
+            m = dev.measure()
+        else:
+            saved = self._fetch() if self._dev_state_valid else None
+            self._upload(phi)
+            m = dev.measure()
+            if saved is not None:
+                self._upload(saved)
+        if self._in_callback and time != self.current_time:
+            dev.set_potential(0, self.getVtotal(self.current_time))   # the running block's potential
         return m
 
     # ------------------------------------------------------------------ solvers
@@ -447,10 +451,12 @@ class GPEtaskSpinor(GPEtask):
     def _measure_sp(self, phi, time):
         self._check_omega()
         dev = self._device()
-        if self._resident(phi, time):
+        if self._in_callback:
             dev.set_potential_mu(0, None)   # the step potential at current_time; the next block-end
-            return dev.measure()            # potential is set again before it is used
-        self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+            if self._resident(phi, time):   # potential is set again before it is used
+                return dev.measure()
+        else:
+            self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
         if time is None:
             time = self.current_time
         self._upload_potential(self.getVtotal(time))
@@ -464,6 +470,8 @@ class GPEtaskSpinor(GPEtask):
         m = dev.measure()
         if saved is not None:
             self._upload(saved)
+        if self._in_callback and time != self.current_time:
+            self._upload_potential(self.getVtotal(self.current_time))   # the running block's potential
         return m
 
     def solve_split(self, ddt, niter_inner, niter_outer):

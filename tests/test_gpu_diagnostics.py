@@ -157,10 +157,16 @@ def test_spinor_callback_energy_with_time_dependent_potential():
         t.Ntotal = 4.0
         t.init_state = [p.copy() for p in psi]
         t.Vtd = vtd
-        t.user_callback = lambda task: rec[id(task)].append(task.get_energy())
+        t.user_callback = lambda task: rec[id(task)].append((task.get_energy(), task.get_energy(task.current_state)))
     pg, po = tg.solve_split(0.004, 20, 3), to.solve_split(0.004, 20, 3)
     assert max(rel_l2(pg[0], po[0]), rel_l2(pg[1], po[1])) < 1e-11
-    assert np.allclose(rec[id(tg)], rec[id(to)], rtol=1e-11, atol=0)
+    assert np.allclose(np.array(rec[id(tg)]), np.array(rec[id(to)]), rtol=1e-11, atol=0)
+    # the run without a callback ends in the same state, bit for bit
+    tq = gpe_hip.GPEtaskSpinor(G, [V, V + 0.1 * Y])
+    tq.g, tq.coupling, tq.Ntotal, tq.Vtd = tg.g, tg.coupling, 4.0, vtd
+    tq.init_state = [p.copy() for p in psi]
+    pq = tq.solve_split(0.004, 20, 3)
+    assert np.array_equal(pq[0], pg[0]) and np.array_equal(pq[1], pg[1])
     got = tg.detect_core(refine=1, comp=0)
     want = oracle.detect_core(po[0], O.mesh.x, O.mesh.y, 1)
     _same_cores(got, want, "spinor component 0")
@@ -176,3 +182,28 @@ def test_detect_core_rejects_bad_input():
         gpe_hip.detect_core(np.ones((9, 9), complex), X, Y)
     with pytest.raises(ValueError, match="meshgrid"):
         gpe_hip.detect_core(np.ones((9, 9), complex), Y, X)
+
+
+@pytest.mark.parametrize("Ntotal", [0.0, 50.0])
+def test_energies_of_other_states_and_times_inside_callback_keep_the_run(Ntotal):
+    # get_energy(phi) and get_energy(time=t) inside user_callback upload what they need
+    # and put the running block's state and potential back; Ntotal = 0 takes the norm of
+    # the initial state (solve_split.m:17-21), which must not reset the run either
+    def make():
+        tg, to = _vortex_tasks()
+        X = tg.grid.mesh.x
+        for t in (tg, to):
+            t.Ntotal = Ntotal
+            t.Vtd = lambda task, tt: 0.3 * np.sin(3.0 * tt) * X    # noqa: B023
+        return tg, to
+
+    tg, to = make()
+    rec = {id(tg): [], id(to): []}
+    for t in (tg, to):
+        t.user_callback = lambda task: rec[id(task)].append(
+            (task.get_energy(task.current_state), task.get_energy(None, 0.0), task.get_energy()))
+    pg, po = tg.solve_split(0.004, 20, 3), to.solve_split(0.004, 20, 3)
+    assert rel_l2(pg, po) < 1e-11
+    assert np.allclose(np.array(rec[id(tg)]), np.array(rec[id(to)]), rtol=1e-11, atol=0)
+    tq, _ = make()
+    assert np.array_equal(tq.solve_split(0.004, 20, 3), pg)
