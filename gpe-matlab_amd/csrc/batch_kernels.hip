@@ -20,7 +20,8 @@ __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
   __shared__ double red[4];
-  const WorkItem it = b.items[blockIdx.x];
+  // 8-pencil tiles: consecutive items (neighbouring tiles of one trajectory) on one XCD
+  const WorkItem it = b.items[C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x];
   const TrajDev& tj = b.traj[it.traj];
   SweepParams p{};
   p.psi = tj.psi;

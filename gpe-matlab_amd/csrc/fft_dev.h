@@ -182,7 +182,7 @@ GPE_PLAN(96, 12, 3, 12, 4, 2, 1)
 GPE_PLAN(128, 16, 2, 16, 8, 1, 1)
 GPE_PLAN(192, 12, 3, 12, 4, 4, 1)
 GPE_PLAN(256, 16, 2, 16, 16, 1, 1)
-GPE_PLAN(384, 24, 3, 12, 8, 4, 1)
+GPE_PLAN(384, 12, 4, 12, 4, 4, 2)
 GPE_PLAN(512, 16, 3, 16, 16, 2, 1)
 #undef GPE_PLAN
 
