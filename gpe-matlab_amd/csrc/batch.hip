@@ -1,9 +1,11 @@
 // Batched ragged ensemble driver: many independent 2D GPEtask.solve_split runs
 // (different grid sizes, potentials, g, gamma) advanced together.
 //
-// Every sweep of a block is one launch per distinct pencil length across all
-// trajectories (work items = (trajectory, pencil tile)), so a block of n_rec
-// steps costs 2 n_rec + 5 launches per length group instead of per trajectory.
+// Every sweep of a block is one launch per distinct pencil length across the
+// trajectories of a chain (work items = (trajectory, pencil tile)), so a block of
+// n_rec steps costs 2 n_rec + 5 launches per length group instead of per
+// trajectory; chains (trajectories of different grid shapes) run on up to four
+// streams at once.
 // With decay_rate = 0 everywhere, one whole outer iteration (niter_inner steps)
 // is captured once into a hipGraph and replayed.  Per-trajectory scalars (mu,
 // renormalisation, ncur) live in a DevState array and are reduced on the device.
@@ -18,9 +20,10 @@ using namespace gpe_detail;
 
 namespace gpe {
 
-__global__ void __launch_bounds__(256) batch_finalize_kernel(const TrajDev* traj, int mode, double time2) {
+__global__ void __launch_bounds__(256) batch_finalize_kernel(const TrajDev* traj, const int* list, int mode,
+                                                             double time2) {
   __shared__ double red[4];
-  const TrajDev& tj = traj[blockIdx.x];
+  const TrajDev& tj = traj[list[blockIdx.x]];
   FinParams f{};
   f.partials = tj.partials;
   f.maxwg = tj.maxwg;
@@ -37,8 +40,9 @@ __global__ void __launch_bounds__(256) batch_finalize_kernel(const TrajDev* traj
   finalize_core(f, red);
 }
 
-hipError_t launch_batch_finalize(const TrajDev* traj, int ntraj, int mode, double time2, hipStream_t s) {
-  hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)ntraj), dim3(256), 0, s, traj, mode, time2);
+hipError_t launch_batch_finalize(const TrajDev* traj, const int* list, int ntraj, int mode, double time2,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)ntraj), dim3(256), 0, s, traj, list, mode, time2);
   return hipGetLastError();
 }
 
@@ -75,6 +79,16 @@ struct gpe_batch {
     size_t first = 0;  // offset into the device item array
     int count = 0;
   };
+  // Trajectories of different shapes never interact, so they advance on up to four
+  // streams at once: the launches of one shape are small and would leave the GPU
+  // partly idle between them.  Each chain owns its trajectories' work items.
+  struct Chain {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<Group> gx, gy;
+    std::vector<int> traj;
+    int* d_list = nullptr;   // traj on the device (finalize)
+  };
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<Traj> tr;
@@ -92,7 +106,8 @@ struct gpe_batch {
   gpe::TrajDev* d_traj = nullptr;
   std::vector<gpe::TrajDev> h_traj;
   gpe::WorkItem* d_items = nullptr;
-  std::vector<Group> gx, gy;
+  std::vector<Chain> ch;
+  hipEvent_t fork = nullptr;
   double ddt = 0;
   int niter_inner = 0, n_rec = 1;
   bool begun = false;
@@ -121,43 +136,60 @@ int bfail(gpe_batch* b, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail((b), GPE_EHIP, "%s: %s", #x, hipGetErrorString(e_));   \
   } while (0)
 
-int bsweep(gpe_batch* b, bool xaxis, int seq) {
-  for (const auto& g : (xaxis ? b->gx : b->gy)) {
+int bsweep(gpe_batch* b, const gpe_batch::Chain& c, bool xaxis, int seq) {
+  for (const auto& g : (xaxis ? c.gx : c.gy)) {
     gpe::BatchLaunch L{b->d_traj, b->d_items + g.first, g.count};
-    hipError_t e = gpe::launch_batch_sweep(g.n, xaxis, seq, L, b->stream);
+    hipError_t e = gpe::launch_batch_sweep(g.n, xaxis, seq, L, c.stream);
     if (e != hipSuccess) return bfail(b, GPE_EHIP, "batch sweep (n %d, seq %d): %s", g.n, seq, hipGetErrorString(e));
   }
   return GPE_OK;
 }
 
-int bmeasure(gpe_batch* b, int mode, double time2) {
+int bmeasure(gpe_batch* b, const gpe_batch::Chain& c, int mode, double time2) {
   int rc;
-  if ((rc = bsweep(b, true, gpe::S_KEX))) return rc;
-  if ((rc = bsweep(b, false, gpe::S_KE))) return rc;
-  BCHK(b, gpe::launch_batch_finalize(b->d_traj, (int)b->tr.size(), mode, time2, b->stream));
+  if ((rc = bsweep(b, c, true, gpe::S_KEX))) return rc;
+  if ((rc = bsweep(b, c, false, gpe::S_KE))) return rc;
+  BCHK(b, gpe::launch_batch_finalize(b->d_traj, c.d_list, (int)c.traj.size(), mode, time2, c.stream));
   return GPE_OK;
 }
 
-// solve_split.m:53-78 for every trajectory at once
-int bblock(gpe_batch* b, double time2) {
+// solve_split.m:53-78 for every trajectory of one chain
+int bblock(gpe_batch* b, const gpe_batch::Chain& c, double time2) {
   int rc;
   for (int i = 0; i < b->n_rec; ++i) {
-    if ((rc = bsweep(b, true, i == 0 ? gpe::S_OPEN : gpe::S_MID))) return rc;
-    if ((rc = bsweep(b, false, gpe::S_KIN))) return rc;
+    if ((rc = bsweep(b, c, true, i == 0 ? gpe::S_OPEN : gpe::S_MID))) return rc;
+    if ((rc = bsweep(b, c, false, gpe::S_KIN))) return rc;
   }
-  if ((rc = bsweep(b, true, gpe::S_CLOSE))) return rc;
-  return bmeasure(b, gpe::FIN_BLOCK, time2);
+  if ((rc = bsweep(b, c, true, gpe::S_CLOSE))) return rc;
+  return bmeasure(b, c, gpe::FIN_BLOCK, time2);
+}
+
+// Runs fn(chain) on every chain's stream between a fork from and a join into b->stream
+// (also inside a graph capture of b->stream, where it records parallel branches).
+template <class F>
+int on_chains(gpe_batch* b, F fn) {
+  BCHK(b, hipEventRecord(b->fork, b->stream));
+  for (auto& c : b->ch) {
+    BCHK(b, hipStreamWaitEvent(c.stream, b->fork, 0));
+    int rc;
+    if ((rc = fn(c))) return rc;
+    BCHK(b, hipEventRecord(c.done, c.stream));
+  }
+  for (auto& c : b->ch) BCHK(b, hipStreamWaitEvent(b->stream, c.done, 0));
+  return GPE_OK;
 }
 
 int bouter(gpe_batch* b, int j) {
-  int rc;
   const double dt_outer = b->ddt * b->niter_inner;
   const double time = (double)(j - 1) * dt_outer;
-  for (int jj = 1; jj <= b->niter_inner / b->n_rec; ++jj) {
-    const double time2 = time + (double)(jj - 1) * b->ddt * (double)b->n_rec;
-    if ((rc = bblock(b, time2))) return rc;
-  }
-  return GPE_OK;
+  return on_chains(b, [&](const gpe_batch::Chain& c) {
+    int rc;
+    for (int jj = 1; jj <= b->niter_inner / b->n_rec; ++jj) {
+      const double time2 = time + (double)(jj - 1) * b->ddt * (double)b->n_rec;
+      if ((rc = bblock(b, c, time2))) return rc;
+    }
+    return GPE_OK;
+  });
 }
 
 void drop_graph(gpe_batch* b) {
@@ -189,6 +221,15 @@ int gpe_batch_destroy(gpe_batch* b) {
   (void)hipFree(b->partials);
   (void)hipFree(b->d_traj);
   (void)hipFree(b->d_items);
+  for (auto& c : b->ch) {
+    if (c.stream) {
+      (void)hipStreamSynchronize(c.stream);
+      (void)hipStreamDestroy(c.stream);
+    }
+    if (c.done) (void)hipEventDestroy(c.done);
+    (void)hipFree(c.d_list);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return GPE_OK;
@@ -234,23 +275,43 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     maxwg = std::max(maxwg, std::max(t.tiles_x, t.tiles_y));
   }
   b->maxwg = maxwg;
-  // work items grouped by swept length
+  // chains: one per grid shape, the smallest shapes (by points) merged until at most
+  // four remain; in a chain, work items grouped by swept length
+  std::map<std::pair<int, int>, std::vector<int>> by_shape;
+  for (int i = 0; i < ntraj; ++i) by_shape[{b->tr[i].nx, b->tr[i].ny}].push_back(i);
+  std::vector<std::pair<double, std::vector<int>>> bins;
+  for (auto& kv : by_shape) bins.push_back({(double)kv.first.first * kv.first.second * kv.second.size(), kv.second});
+  std::sort(bins.begin(), bins.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
+  while (bins.size() > 4) {
+    auto last = bins.back();
+    bins.pop_back();
+    auto& into = bins.back();   // the smallest remaining chain takes the smallest shape
+    into.first += last.first;
+    into.second.insert(into.second.end(), last.second.begin(), last.second.end());
+    std::sort(into.second.begin(), into.second.end());
+    std::sort(bins.begin(), bins.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
+  }
+  b->ch.resize(bins.size());
   std::vector<gpe::WorkItem> items;
-  for (int axis = 0; axis < 2; ++axis) {
-    std::map<int, std::vector<gpe::WorkItem>> by_n;
-    for (int i = 0; i < ntraj; ++i) {
-      const auto& t = b->tr[i];
-      const int n = axis == 0 ? t.nx : t.ny;
-      const int tiles = axis == 0 ? t.tiles_x : t.tiles_y;
-      for (int k = 0; k < tiles; ++k) by_n[n].push_back({i, k});
-    }
-    for (auto& kv : by_n) {
-      gpe_batch::Group g;
-      g.n = kv.first;
-      g.first = items.size();
-      g.count = (int)kv.second.size();
-      items.insert(items.end(), kv.second.begin(), kv.second.end());
-      (axis == 0 ? b->gx : b->gy).push_back(g);
+  for (size_t ci = 0; ci < bins.size(); ++ci) {
+    auto& c = b->ch[ci];
+    c.traj = bins[ci].second;
+    for (int axis = 0; axis < 2; ++axis) {
+      std::map<int, std::vector<gpe::WorkItem>> by_n;
+      for (int i : c.traj) {
+        const auto& t = b->tr[i];
+        const int n = axis == 0 ? t.nx : t.ny;
+        const int tiles = axis == 0 ? t.tiles_x : t.tiles_y;
+        for (int k = 0; k < tiles; ++k) by_n[n].push_back({i, k});
+      }
+      for (auto& kv : by_n) {
+        gpe_batch::Group g;
+        g.n = kv.first;
+        g.first = items.size();
+        g.count = (int)kv.second.size();
+        items.insert(items.end(), kv.second.begin(), kv.second.end());
+        (axis == 0 ? c.gx : c.gy).push_back(g);
+      }
     }
   }
   auto bail = [&](int code) {
@@ -258,6 +319,14 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     return code;
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(GPE_EHIP);
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bail(GPE_EHIP);
+  for (auto& c : b->ch) {
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(GPE_EHIP);
+    if (hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess) return bail(GPE_EHIP);
+    if (hipMalloc(&c.d_list, c.traj.size() * sizeof(int)) != hipSuccess) return bail(GPE_ENOMEM);
+    if (hipMemcpy(c.d_list, c.traj.data(), c.traj.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(GPE_EHIP);
+  }
   if (hipMalloc(&b->psi, b->total * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
   if (hipMalloc(&b->V, b->total * sizeof(double)) != hipSuccess) return bail(GPE_ENOMEM);
   if (hipMalloc(&b->kcol, b->ktotal_x * sizeof(double2)) != hipSuccess) return bail(GPE_ENOMEM);
@@ -408,7 +477,7 @@ int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_gra
   BCHK(b, hipMemcpy(b->st, s.data(), ntraj * sizeof(DevState), hipMemcpyHostToDevice));
   // solve_split.m:40-46: initial muc of every trajectory
   int rc;
-  if ((rc = bmeasure(b, gpe::FIN_SUMS, 0.0))) return rc;
+  if ((rc = on_chains(b, [&](const gpe_batch::Chain& c) { return bmeasure(b, c, gpe::FIN_SUMS, 0.0); }))) return rc;
   BCHK(b, hipStreamSynchronize(b->stream));
   BCHK(b, hipMemcpy(s.data(), b->st, ntraj * sizeof(DevState), hipMemcpyDeviceToHost));
   for (int i = 0; i < ntraj; ++i) {

@@ -70,3 +70,17 @@ def test_ensemble_matches_oracle_and_resumes():
         print(f"ensemble traj {i} {G.size}: rel-L2 vs oracle {err:.2e}")
         assert err < 1e-11
         assert np.allclose(ens_tasks[i].history["mu"], to.history["mu"], rtol=1e-11, atol=0)
+
+
+def test_ensemble_with_long_pencils_matches_single_runs():
+    # 384- and 512-point axes run 8-pencil tiles (paired on one XCD), several
+    # trajectories per shape share a chain
+    import gpe_hip
+    from gpe_hip.batch import GPEEnsemble
+    shapes = [(384, 32), (32, 384), (512, 24), (24, 512), (384, 32), (512, 24)]
+    specs = _tasks(3, shapes, [0.0, 0.02, 0.0, 0.01, 0.03, 0.0])
+    ens_tasks = [_make(gpe_hip, s) for s in specs]
+    outs = GPEEnsemble(ens_tasks).solve_split(0.004, 20, 2)
+    for i, s in enumerate(specs):
+        ref = _make(gpe_hip, s).solve_split(0.004, 20, 2)
+        assert np.array_equal(outs[i], ref), (i, rel_l2(outs[i], ref))
