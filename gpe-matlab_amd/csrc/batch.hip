@@ -270,7 +270,8 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     b->ktotal_x += t.nx;
     t.koff_y = b->ktotal_y;
     b->ktotal_y += t.ny;
-    t.tiles_x = (t.ny + gpe::sweep_cols_per_block(t.nx) - 1) / gpe::sweep_cols_per_block(t.nx);
+    // x sweeps are phase sweeps (open / mid / close / kex)
+    t.tiles_x = (t.ny + gpe::sweep_cols_per_block_phase(t.nx) - 1) / gpe::sweep_cols_per_block_phase(t.nx);
     t.tiles_y = (t.nx + gpe::sweep_cols_per_block(t.ny) - 1) / gpe::sweep_cols_per_block(t.ny);
     maxwg = std::max(maxwg, std::max(t.tiles_x, t.tiles_y));
   }

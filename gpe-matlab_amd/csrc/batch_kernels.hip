@@ -16,7 +16,7 @@ namespace gpe {
 
 template <int N, bool COLFAST, int SEQ>
 __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
-  constexpr int C = 256 / Plan<N>::T;
+  constexpr int C = sweep_cols<N, SEQ>();
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
   __shared__ double red[4];

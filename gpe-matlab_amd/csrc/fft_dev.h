@@ -186,6 +186,18 @@ GPE_PLAN(384, 12, 4, 12, 4, 4, 2)
 GPE_PLAN(512, 16, 3, 16, 16, 2, 1)
 #undef GPE_PLAN
 
+// 256 points as 8.8.4 with 8 elements per thread (32 threads per pencil): twice the
+// LDS exchanges of 16.16, half the registers.  The x sweeps that carry the phase
+// step use it (see SweepPlan): their exp/sincos work then runs at 4 waves per SIMD
+// instead of 2 (mid_x 118 -> 110 us at 256^3); the transform-only and kinetic sweeps
+// keep 16.16, where the extra exchanges cost more than the occupancy returns.
+struct Plan256x8 {
+  static constexpr int N = 256, E = 8, T = 32, P = 3;
+  static constexpr int R0 = 8, R1 = 8, R2 = 4, R3 = 1;
+  __device__ __host__ static constexpr int radix(int p) { return p == 0 ? 8 : p == 1 ? 8 : p == 2 ? 4 : 1; }
+  __device__ __host__ static constexpr int ns(int p) { return p == 0 ? 1 : p == 1 ? 8 : p == 2 ? 64 : 256; }
+};
+
 // One Stockham pass on the canonical layout.  Butterfly j = t + T*b takes
 // elements j + r*N/R (register b + r*B), twiddles them by W_{NS R}^{r (j mod NS)}
 // and applies a radix-R DFT; output r of butterfly j belongs at

@@ -78,11 +78,11 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   h->ax[2].stride = nx * ny; h->ax[2].ncols = nx * ny; h->ax[2].A = nx * ny; h->ax[2].sa = 1; h->ax[2].sb = 0;
 
   int maxwg = 2048;
-  for (int a = 0; a < h->ndims; ++a) {
-    const int C = gpe::sweep_cols_per_block(nn[a]);
-    const long long nb = (h->ax[a].ncols + C - 1) / C;
-    if (nb > maxwg) maxwg = (int)nb;
-  }
+  for (int a = 0; a < h->ndims; ++a)
+    for (int C : {gpe::sweep_cols_per_block(nn[a]), gpe::sweep_cols_per_block_phase(nn[a])}) {
+      const long long nb = (h->ax[a].ncols + C - 1) / C;
+      if (nb > maxwg) maxwg = (int)nb;
+    }
   {  // two-component x-sweeps put C/2 columns in a work-group (one lane per component)
     const int C2 = gpe::sweep_cols_per_block(nn[0]) / 2;
     const long long nb = (h->ax[0].ncols + C2 - 1) / C2;

@@ -39,6 +39,8 @@ hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st
 hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s,
                         bool reset = true);
 int sweep_cols_per_block(int n);
+// the same for the phase sweeps (open / mid / close / kex / ITP on the x axis)
+int sweep_cols_per_block_phase(int n);
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
 
 }  // namespace gpe

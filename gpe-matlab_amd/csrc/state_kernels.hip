@@ -5,6 +5,7 @@
 #include "reduce_dev.h"
 #include "finalize_dev.h"
 #include "state.h"
+#include "sweep_body.h"
 
 namespace gpe {
 
@@ -88,6 +89,10 @@ bool sweep_length_supported(int n) {
     default:
       return false;
   }
+}
+
+int sweep_cols_per_block_phase(int n) {
+  return n == 256 ? sweep_cols<256, S_MID>() : sweep_cols_per_block(n);
 }
 
 int sweep_cols_per_block(int n) {

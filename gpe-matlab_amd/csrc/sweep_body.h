@@ -6,6 +6,8 @@
 // (the library limits a grid to 2^27 points), inactive tail pencils load a
 // valid pencil and skip the store, and the twiddle table is staged in LDS.
 #pragma once
+#include <type_traits>
+
 #include "fft_dev.h"
 #include "phase_dev.h"
 #include "philox_dev.h"
@@ -13,6 +15,20 @@
 #include "sweep.h"
 
 namespace gpe {
+
+// Sequences that carry the phase step (they run on the x axis).
+constexpr bool phase_seq(int s) {
+  return s == S_OPEN || s == S_MID || s == S_CLOSE || s == S_CLOSEM || s == S_KEX || s == S_ITPA || s == S_ITPB ||
+         s == S_ROPEN || s == S_RMID || s == S_RCLOSEM || s == S_RITPA || s == S_RITPB || s == S_SOPEN ||
+         s == S_SMID;
+}
+
+// FFT plan of a sweep: Plan<N>, except 8.8.4 for the phase sweeps of 256-point pencils.
+template <int N, int SEQ> struct SweepPlan { using type = Plan<N>; };
+template <int SEQ> struct SweepPlan<256, SEQ> {
+  using type = typename std::conditional<phase_seq(SEQ), Plan256x8, Plan<256>>::type;
+};
+template <int N, int SEQ> constexpr int sweep_cols() { return 256 / SweepPlan<N, SEQ>::type::T; }
 
 template <int SEQ> struct SeqTraits {
   static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM ||
@@ -47,7 +63,7 @@ __device__ __forceinline__ double ke_weight(const SweepParams& p, int i, double 
 template <int N, int C, bool COLFAST, int SEQ>
 __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, unsigned wg, double* lds,
                                            double2* twl, double* red) {
-  using PL = Plan<N>;
+  using PL = typename SweepPlan<N, SEQ>::type;
   using TR = SeqTraits<SEQ>;
   constexpr int T = PL::T, E = PL::E;
   static_assert(C * T == 256, "work-group is 256 threads");

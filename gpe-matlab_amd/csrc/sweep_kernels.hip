@@ -14,15 +14,16 @@ namespace gpe {
 hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, const SweepParams& p, hipStream_t s,
                                           int* nblocks) {
   constexpr int N = GPE_N;
-  constexpr int C = 256 / Plan<N>::T;
-  const long long nb = ((long long)p.ncols + C - 1) / C;
-  if (nblocks) *nblocks = (int)nb;
-  if (nb <= 0) return hipSuccess;
-  dim3 grid((unsigned)nb), block(256);
-#define GPE_L(CF, S)                                                       \
-  case S:                                                                  \
-    hipLaunchKernelGGL((sweep_kernel<N, C, CF, S>), grid, block, 0, s, p); \
-    break;
+  long long nb = 0;
+  const dim3 block(256);
+  // pencils per work-group follow the sequence's plan (sweep_cols)
+#define GPE_L(CF, S)                                                                                   \
+  case S: {                                                                                            \
+    constexpr int C = sweep_cols<N, S>();                                                              \
+    nb = ((long long)p.ncols + C - 1) / C;                                                             \
+    if (nb > 0) hipLaunchKernelGGL((sweep_kernel<N, C, CF, S>), dim3((unsigned)nb), block, 0, s, p);   \
+    break;                                                                                             \
+  }
   if (colfast) {
     switch (seq) {
       GPE_L(true, S_FWD) GPE_L(true, S_INV) GPE_L(true, S_KIN) GPE_L(true, S_OPEN) GPE_L(true, S_MID)
@@ -39,7 +40,8 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, const SweepParam
     }
   }
 #undef GPE_L
-  return hipGetLastError();
+  if (nblocks) *nblocks = (int)nb;
+  return nb > 0 ? hipGetLastError() : hipSuccess;
 }
 
 }  // namespace gpe
