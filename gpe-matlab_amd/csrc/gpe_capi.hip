@@ -132,10 +132,20 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   return GPE_OK;
 }
 
+namespace {
+void drop_outer_graph(gpe_handle* h) {
+  if (h->ogx) (void)hipGraphExecDestroy(h->ogx);
+  if (h->og) (void)hipGraphDestroy(h->og);
+  h->ogx = nullptr;
+  h->og = nullptr;
+}
+}  // namespace
+
 int gpe_destroy(gpe_handle* h) {
   if (!h) return GPE_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
+  drop_outer_graph(h);
   for (int c = 0; c < 2; ++c) {
     (void)hipFree(h->psi[c]);
     (void)hipFree(h->V[c]);
@@ -297,8 +307,8 @@ int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
 int gpe_set_omega(gpe_handle* h, double omega) {
   if (!h) return GPE_EINVAL;
   if (!std::isfinite(omega)) return fail(h, GPE_EINVAL, "omega must be finite");
+  if (omega != h->omega) h->begun = false;   // the kinetic tables change: gpe_split_begin first
   h->omega = omega;
-  h->begun = false;   // the kinetic tables change: the next solve_split starts with gpe_split_begin
   return GPE_OK;
 }
 
@@ -333,6 +343,11 @@ int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out)
     return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
   }
   if ((rc = materialize(h))) return rc;
+  drop_outer_graph(h);
+  {
+    const char* ng = getenv("GPE_NO_GRAPH");
+    h->og_off = ng && atoi(ng) != 0;
+  }
   double m[5];
   if ((rc = gpe_measure(h, m))) return rc;
   // solve_split.m:17-21: NN0 = Ntotal, or the norm of the state
@@ -382,9 +397,33 @@ int gpe_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, double
   const double time = (double)(j - 1) * dt_outer;               // solve_split.m:50
   double time2 = time;
   const int nblocks = h->niter_inner / h->n_rec;
-  for (int jj = 1; jj <= nblocks; ++jj) {
-    time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
-    if ((rc = run_block(h, time2))) return rc;
+  // the blocks of an outer iteration differ only in time2, which enters the arithmetic
+  // only through decay_rate: without it the iteration is captured once and replayed
+  const bool graph = !h->og_off && !h->prof && h->prm.decay_rate == 0;
+  if (graph && !h->ogx) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int crc = GPE_OK;
+    for (int jj = 1; jj <= nblocks && !crc; ++jj) crc = run_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (!crc && ec == hipSuccess && g && hipGraphInstantiate(&h->ogx, g, nullptr, nullptr, 0) == hipSuccess) {
+      h->og = g;
+    } else {
+      if (g) (void)hipGraphDestroy(g);
+      h->ogx = nullptr;
+      h->og_off = true;   // run the blocks directly from now on
+      (void)hipGetLastError();
+      if (crc) return crc;
+    }
+  }
+  if (graph && h->ogx) {
+    HIPCHK(h, hipGraphLaunch(h->ogx, h->stream));
+    time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
+  } else {
+    for (int jj = 1; jj <= nblocks; ++jj) {
+      time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
+      if ((rc = run_block(h, time2))) return rc;
+    }
   }
   DevState s;
   if ((rc = read_state(h, &s))) return rc;

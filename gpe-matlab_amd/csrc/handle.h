@@ -120,6 +120,11 @@ struct gpe_handle {
   size_t dc_bytes = 0;
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
+  // one outer iteration of gpe_split_outer captured as a graph (static potential,
+  // decay_rate = 0); dropped by gpe_split_begin
+  hipGraph_t og = nullptr;
+  hipGraphExec_t ogx = nullptr;
+  bool og_off = false;   // capture failed or disabled (GPE_NO_GRAPH)
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;

@@ -211,3 +211,19 @@ def test_device_phase_accuracy():
     # NaN/Inf arguments propagate as NaN; a hugely negative real part underflows to 0
     sp = selftest_phase(np.array([np.nan, 0.0, -1000.0]), np.array([0.0, np.inf, 1.0]))
     assert np.isnan(sp[0].real) and np.isnan(sp[1].real) and sp[2] == 0
+
+
+@pytest.mark.parametrize("dims,n,xmax", [(2, (64, 48), (8, 7)), (3, (24, 16, 32), (6, 5, 7))])
+def test_outer_graph_replays_equal_direct_launches(dims, n, xmax, monkeypatch):
+    # gpe_split_outer captures one outer iteration as a graph and replays it; every
+    # replay (with eager work in the callback between them) equals the direct launches
+    seen = {"graph": [], "direct": []}
+    for mode in ("graph", "direct"):
+        if mode == "direct":
+            monkeypatch.setenv("GPE_NO_GRAPH", "1")
+        tg, _ = _tasks(dims, n, xmax, harm2 if dims == 2 else harm3, gamma=0.02)
+        tg.user_callback = lambda task: seen[mode].append((task.current_state.copy(), task.get_energy()))  # noqa: B023
+        tg.solve_split(0.004, 20, 4)
+    assert len(seen["graph"]) == len(seen["direct"]) == 4
+    for (a, ea), (b, eb) in zip(seen["graph"], seen["direct"]):
+        assert np.array_equal(a, b) and ea == eb
