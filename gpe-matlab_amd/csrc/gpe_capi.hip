@@ -132,15 +132,6 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   return GPE_OK;
 }
 
-namespace {
-void drop_outer_graph(gpe_handle* h) {
-  if (h->ogx) (void)hipGraphExecDestroy(h->ogx);
-  if (h->og) (void)hipGraphDestroy(h->og);
-  h->ogx = nullptr;
-  h->og = nullptr;
-}
-}  // namespace
-
 int gpe_destroy(gpe_handle* h) {
   if (!h) return GPE_OK;
   (void)hipSetDevice(h->device);
@@ -338,16 +329,16 @@ int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out)
   int rc;
   if ((rc = ensure_ready(h))) return rc;
   if (niter_inner < 1) return fail(h, GPE_EINVAL, "niter_inner must be >= 1");
+  drop_outer_graph(h);   // tables and parameters may change below
+  {
+    const char* ng = getenv("GPE_NO_GRAPH");
+    h->og_off = ng && atoi(ng) != 0;
+  }
   if (h->prm.ncomp == 2) {
     if (h->omega != 0.0) return fail(h, GPE_EINVAL, "the rotating frame is supported for one component");
     return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
   }
   if ((rc = materialize(h))) return rc;
-  drop_outer_graph(h);
-  {
-    const char* ng = getenv("GPE_NO_GRAPH");
-    h->og_off = ng && atoi(ng) != 0;
-  }
   double m[5];
   if ((rc = gpe_measure(h, m))) return rc;
   // solve_split.m:17-21: NN0 = Ntotal, or the norm of the state
@@ -399,32 +390,16 @@ int gpe_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, double
   const int nblocks = h->niter_inner / h->n_rec;
   // the blocks of an outer iteration differ only in time2, which enters the arithmetic
   // only through decay_rate: without it the iteration is captured once and replayed
-  const bool graph = !h->og_off && !h->prof && h->prm.decay_rate == 0;
-  if (graph && !h->ogx) {
-    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int crc = GPE_OK;
-    for (int jj = 1; jj <= nblocks && !crc; ++jj) crc = run_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
-    if (!crc && ec == hipSuccess && g && hipGraphInstantiate(&h->ogx, g, nullptr, nullptr, 0) == hipSuccess) {
-      h->og = g;
-    } else {
-      if (g) (void)hipGraphDestroy(g);
-      h->ogx = nullptr;
-      h->og_off = true;   // run the blocks directly from now on
-      (void)hipGetLastError();
-      if (crc) return crc;
-    }
-  }
-  if (graph && h->ogx) {
-    HIPCHK(h, hipGraphLaunch(h->ogx, h->stream));
-    time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
-  } else {
+  const double key[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // every baked-in value is fixed until gpe_split_begin
+  rc = run_outer(h, h->prm.decay_rate == 0, key, [&] {
     for (int jj = 1; jj <= nblocks; ++jj) {
-      time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
-      if ((rc = run_block(h, time2))) return rc;
+      const int r = run_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec);   // 53
+      if (r) return r;
     }
-  }
+    return GPE_OK;
+  });
+  if (rc) return rc;
+  time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
   if (time2_out) *time2_out = time2;

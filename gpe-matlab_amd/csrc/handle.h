@@ -125,6 +125,7 @@ struct gpe_handle {
   hipGraph_t og = nullptr;
   hipGraphExec_t ogx = nullptr;
   bool og_off = false;   // capture failed or disabled (GPE_NO_GRAPH)
+  double og_key[8] = {};  // host values baked into the captured launches
   // per-sweep HIP-event profiling (gpe_profile_enable / gpe_profile_read)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -490,6 +491,41 @@ inline int ensure_ready(gpe_handle* h) {
   for (int c = 0; c < h->prm.ncomp; ++c)
     if (!h->have_V[c]) return fail(h, GPE_ESTATE, "potential of component %d not set", c);
   HIPCHK(h, hipSetDevice(h->device));
+  return GPE_OK;
+}
+
+inline void drop_outer_graph(gpe_handle* h) {
+  if (h->ogx) (void)hipGraphExecDestroy(h->ogx);
+  if (h->og) (void)hipGraphDestroy(h->og);
+  h->ogx = nullptr;
+  h->og = nullptr;
+}
+
+// Runs `launches` (the kernels of one outer iteration, no host synchronisation) through
+// a captured graph when `graphable`: the first call captures, later calls replay while
+// the host values in key[0..7] that the launches bake in are unchanged (else recapture).
+// Profiling, GPE_NO_GRAPH=1 and a failed capture run the launches directly.
+template <class F>
+inline int run_outer(gpe_handle* h, bool graphable, const double (&key)[8], F launches) {
+  if (!graphable || h->og_off || h->prof) return launches();
+  if (h->ogx && std::memcmp(h->og_key, key, sizeof key) != 0) drop_outer_graph(h);
+  if (!h->ogx) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int crc = launches();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (!crc && ec == hipSuccess && g && hipGraphInstantiate(&h->ogx, g, nullptr, nullptr, 0) == hipSuccess) {
+      h->og = g;
+      std::memcpy(h->og_key, key, sizeof key);
+    } else {
+      if (g) (void)hipGraphDestroy(g);
+      h->ogx = nullptr;
+      h->og_off = true;
+      (void)hipGetLastError();
+      return crc ? crc : launches();
+    }
+  }
+  HIPCHK(h, hipGraphLaunch(h->ogx, h->stream));
   return GPE_OK;
 }
 

@@ -244,10 +244,19 @@ int spinor_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, dou
   const double time = (double)(j - 1) * dt_outer;             // 66
   double time2 = time;
   const int nblocks = h->niter_inner / h->n_rec;              // 67 (colon truncates)
-  for (int jj = 1; jj <= nblocks; ++jj) {
-    time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 69
-    if ((rc = spinor_block(h, time2, nullptr, true))) return rc;
-  }
+  // graph replay as in gpe_split_outer; the coupling scalars, its array's magnitude
+  // bound (the Taylor choice) and the block-end potential are baked into the launches
+  const double key[8] = {h->cpl_array ? 1.0 : 0.0, h->cpl_re, h->cpl_im, h->cpl_max2, h->use_vmu ? 1.0 : 0.0,
+                         (double)reinterpret_cast<uintptr_t>(h->cpl), 0.0, 0.0};
+  rc = run_outer(h, h->prm.decay_rate == 0, key, [&] {
+    for (int jj = 1; jj <= nblocks; ++jj) {
+      const int r = spinor_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec, nullptr, true);   // 69
+      if (r) return r;
+    }
+    return GPE_OK;
+  });
+  if (rc) return rc;
+  time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
   if (time2_out) *time2_out = time2;

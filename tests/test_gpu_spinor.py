@@ -75,3 +75,31 @@ def test_spinor_rabi_closed_form_gpu():
     n1 = grid.integrate(np.abs(out[0]) ** 2)
     n2 = grid.integrate(np.abs(out[1]) ** 2)
     assert abs((n1 - n2) / t.Ntotal - np.cos(2 * 0.5 * 1.2)) < 1e-12
+
+
+def test_spinor_outer_graph_replays_equal_direct_launches(monkeypatch):
+    # the two-component outer iteration is replayed from a captured graph; a coupling
+    # changed by the callback is baked into a new capture
+    import gpe_hip
+    seen = {"graph": [], "direct": []}
+    for mode in ("graph", "direct"):
+        if mode == "direct":
+            monkeypatch.setenv("GPE_NO_GRAPH", "1")
+        G = gpe_hip.grid2d(6, 32, 5, 24)
+        r2 = G.mesh.x ** 2 + G.mesh.y ** 2
+        t = gpe_hip.GPEtaskSpinor(G, [0.5 * r2, 0.5 * r2 + 0.2 * G.mesh.y])
+        t.g = np.array([[0.5, 0.45], [0.4, 0.6]])
+        t.coupling = 0.3 * np.exp(0.5j)
+        t.Ntotal = 4.0
+        t.init_state = [np.exp(-r2 / 3) + 0j, 0.7 * np.exp(-r2 / 2) * np.exp(0.2j * G.mesh.x)]
+
+        def cb(task, mode=mode):
+            psi = task.current_state
+            seen[mode].append((psi[0].copy(), psi[1].copy(), task.get_energy()))
+            if task.current_iter == 2:
+                task.coupling = 0.5 * np.exp(-0.2j)
+        t.user_callback = cb
+        t.solve_split(0.004, 20, 4)
+    assert len(seen["graph"]) == len(seen["direct"]) == 4
+    for a, b in zip(seen["graph"], seen["direct"]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
