@@ -123,7 +123,12 @@ def main():
     task = gpe_hip.GPEtask(grid, V)
     task.Ntotal, task.g, task.gamma = props["Ntotal"], props["g"], props["gamma"]
     task.omega = args.omega
+    task.fftw_set_wisdom = False       # tuned explicitly below, to report the measurement
     dev = task._device()
+    tune_us = dev.tune()
+    plans = dev.plans()
+    log("fft plan variants per axis " + str(plans) + ", tuner times (us) [axis][variant] "
+        + str([[round(float(x), 1) for x in r] for r in tune_us]))
     task._set_params(task.Ntotal)
     dev.set_potential(0, V)
     npts = int(np.prod(grid.size))
@@ -236,6 +241,8 @@ def main():
                                + (f", rotating frame omega={args.omega}" if args.omega else ""),
                    "grid": [args.n] * 3, "parallelism": "single GPU"},
         "roofline": roofline,
+        "fft_plans": {"variant_per_axis": plans,
+                      "tune_us": [[None if not np.isfinite(x) else round(float(x), 1) for x in r] for r in tune_us]},
         "step_alg_GBps": round(step_gbps, 1),
         "step_roofline_frac": round(step_gbps / PEAK_HBM_GBPS, 4),
         "parity_rel_l2_vs_oracle": parity,

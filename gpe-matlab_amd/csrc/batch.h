@@ -38,7 +38,8 @@ struct BatchLaunch {
 
 // xaxis: x sweep (strided, phase sequences) or y sweep (contiguous, kinetic / KE)
 // over items whose swept axis has length n
-hipError_t launch_batch_sweep(int n, bool xaxis, int seq, const BatchLaunch& b, hipStream_t s);
+// with tiles of plan variant var
+hipError_t launch_batch_sweep(int n, bool xaxis, int seq, int var, const BatchLaunch& b, hipStream_t s);
 // one work-group per trajectory list[0 .. ntraj): FIN_BLOCK or FIN_SUMS on its partials
 hipError_t launch_batch_finalize(const TrajDev* traj, const int* list, int ntraj, int mode, double time2,
                                  hipStream_t s);

@@ -46,14 +46,14 @@ hipError_t launch_batch_finalize(const TrajDev* traj, const int* list, int ntraj
   return hipGetLastError();
 }
 
-#define GPE_DECL(N) hipError_t launch_batch_sweep_##N(bool, int, const BatchLaunch&, hipStream_t);
+#define GPE_DECL(N) hipError_t launch_batch_sweep_##N(bool, int, int, const BatchLaunch&, hipStream_t);
 GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
 GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
 #undef GPE_DECL
 
-hipError_t launch_batch_sweep(int n, bool xaxis, int seq, const BatchLaunch& b, hipStream_t s) {
+hipError_t launch_batch_sweep(int n, bool xaxis, int seq, int var, const BatchLaunch& b, hipStream_t s) {
   switch (n) {
-#define GPE_C(N) case N: return launch_batch_sweep_##N(xaxis, seq, b, s);
+#define GPE_C(N) case N: return launch_batch_sweep_##N(xaxis, seq, var, b, s);
     GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
     GPE_C(256) GPE_C(384) GPE_C(512)
 #undef GPE_C
@@ -76,6 +76,7 @@ struct gpe_batch {
   };
   struct Group {
     int n = 0;
+    int var = 0;       // plan variant of the group's sweeps
     size_t first = 0;  // offset into the device item array
     int count = 0;
   };
@@ -106,6 +107,8 @@ struct gpe_batch {
   gpe::TrajDev* d_traj = nullptr;
   std::vector<gpe::TrajDev> h_traj;
   gpe::WorkItem* d_items = nullptr;
+  size_t items_cap = 0;
+  std::map<int, int> var_x, var_y;   // plan variant per pencil length of the x / y sweeps
   std::vector<Chain> ch;
   hipEvent_t fork = nullptr;
   double ddt = 0;
@@ -139,7 +142,7 @@ int bfail(gpe_batch* b, int code, const char* fmt, ...) {
 int bsweep(gpe_batch* b, const gpe_batch::Chain& c, bool xaxis, int seq) {
   for (const auto& g : (xaxis ? c.gx : c.gy)) {
     gpe::BatchLaunch L{b->d_traj, b->d_items + g.first, g.count};
-    hipError_t e = gpe::launch_batch_sweep(g.n, xaxis, seq, L, c.stream);
+    hipError_t e = gpe::launch_batch_sweep(g.n, xaxis, seq, g.var, L, c.stream);
     if (e != hipSuccess) return bfail(b, GPE_EHIP, "batch sweep (n %d, seq %d): %s", g.n, seq, hipGetErrorString(e));
   }
   return GPE_OK;
@@ -190,6 +193,81 @@ int bouter(gpe_batch* b, int j) {
     }
     return GPE_OK;
   });
+}
+
+// Work-group tiles of every trajectory under the current plan variants, the chains'
+// length groups and the device item array.
+int build_items(gpe_batch* b) {
+  for (auto& t : b->tr) {
+    const int vx = b->var_x[t.nx], vy = b->var_y[t.ny];
+    // x sweeps are phase sweeps (open / mid / close / kex)
+    const int cx = gpe::sweep_cols_per_block_phase(t.nx, vx), cy = gpe::sweep_cols_per_block(t.ny, vy);
+    t.tiles_x = (t.ny + cx - 1) / cx;
+    t.tiles_y = (t.nx + cy - 1) / cy;
+  }
+  std::vector<gpe::WorkItem> items;
+  for (auto& c : b->ch) {
+    c.gx.clear();
+    c.gy.clear();
+    for (int axis = 0; axis < 2; ++axis) {
+      std::map<int, std::vector<gpe::WorkItem>> by_n;
+      for (int i : c.traj) {
+        const auto& t = b->tr[i];
+        const int n = axis == 0 ? t.nx : t.ny;
+        const int tiles = axis == 0 ? t.tiles_x : t.tiles_y;
+        for (int k = 0; k < tiles; ++k) by_n[n].push_back({i, k});
+      }
+      for (auto& kv : by_n) {
+        gpe_batch::Group g;
+        g.n = kv.first;
+        g.var = axis == 0 ? b->var_x[g.n] : b->var_y[g.n];
+        g.first = items.size();
+        g.count = (int)kv.second.size();
+        items.insert(items.end(), kv.second.begin(), kv.second.end());
+        (axis == 0 ? c.gx : c.gy).push_back(g);
+      }
+    }
+  }
+  if (items.size() > b->items_cap) {
+    (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    b->items_cap = 0;
+    if (hipMalloc(&b->d_items, items.size() * sizeof(gpe::WorkItem)) != hipSuccess)
+      return bfail(b, GPE_ENOMEM, "work items");
+    b->items_cap = items.size();
+  }
+  BCHK(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(gpe::WorkItem), hipMemcpyHostToDevice));
+  return GPE_OK;
+}
+
+// Device view of trajectory i (pointers, sizes, tiles); the step parameters (g, dt,
+// renormalisation) come from its gpe_params when set.
+gpe::TrajDev traj_dev(gpe_batch* b, int i, std::complex<double> dt) {
+  const auto& t = b->tr[i];
+  gpe::TrajDev d{};
+  d.psi = b->psi + t.off;
+  d.V = b->V + t.off;
+  d.kcol = b->kcol + t.koff_x;
+  d.kax = b->kax + t.koff_y;
+  d.keax_x = b->keax_x + t.koff_x;
+  d.keax_y = b->keax_y + t.koff_y;
+  d.tw_x = b->tw[t.nx];
+  d.tw_y = b->tw[t.ny];
+  d.st = b->st + i;
+  d.partials = b->partials + (size_t)i * gpe::SL_NSLOT * b->maxwg;
+  d.nx = (unsigned)t.nx;
+  d.ny = (unsigned)t.ny;
+  d.maxwg = b->maxwg;
+  d.tiles_x = t.tiles_x;
+  d.tiles_y = t.tiles_y;
+  d.renorm = (t.prm.gamma > 0 && t.prm.mu_init == 0) ? 1 : 0;
+  d.g = t.prm.g[0];
+  d.dr = dt.real();
+  d.di = dt.imag();
+  d.w = t.w;
+  d.NN0 = t.prm.Ntotal;
+  d.tau = t.prm.decay_rate;
+  return d;
 }
 
 void drop_graph(gpe_batch* b) {
@@ -270,10 +348,14 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     b->ktotal_x += t.nx;
     t.koff_y = b->ktotal_y;
     b->ktotal_y += t.ny;
-    // x sweeps are phase sweeps (open / mid / close / kex)
-    t.tiles_x = (t.ny + gpe::sweep_cols_per_block_phase(t.nx) - 1) / gpe::sweep_cols_per_block_phase(t.nx);
-    t.tiles_y = (t.nx + gpe::sweep_cols_per_block(t.ny) - 1) / gpe::sweep_cols_per_block(t.ny);
-    maxwg = std::max(maxwg, std::max(t.tiles_x, t.tiles_y));
+    // partials fit every plan variant
+    const int cx = gpe::sweep_cols_min(t.nx), cy = gpe::sweep_cols_min(t.ny);
+    maxwg = std::max(maxwg, std::max((t.ny + cx - 1) / cx, (t.nx + cy - 1) / cy));
+    for (int a = 0; a < 2; ++a) {
+      const int n = a == 0 ? t.nx : t.ny;
+      const int w = gpe::wisdom_get(2, a, n);
+      (a == 0 ? b->var_x : b->var_y)[n] = w > 0 && w < gpe::plan_variants(n) ? w : 0;
+    }
   }
   b->maxwg = maxwg;
   // chains: one per grid shape, the smallest shapes (by points) merged until at most
@@ -293,28 +375,7 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     std::sort(bins.begin(), bins.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
   }
   b->ch.resize(bins.size());
-  std::vector<gpe::WorkItem> items;
-  for (size_t ci = 0; ci < bins.size(); ++ci) {
-    auto& c = b->ch[ci];
-    c.traj = bins[ci].second;
-    for (int axis = 0; axis < 2; ++axis) {
-      std::map<int, std::vector<gpe::WorkItem>> by_n;
-      for (int i : c.traj) {
-        const auto& t = b->tr[i];
-        const int n = axis == 0 ? t.nx : t.ny;
-        const int tiles = axis == 0 ? t.tiles_x : t.tiles_y;
-        for (int k = 0; k < tiles; ++k) by_n[n].push_back({i, k});
-      }
-      for (auto& kv : by_n) {
-        gpe_batch::Group g;
-        g.n = kv.first;
-        g.first = items.size();
-        g.count = (int)kv.second.size();
-        items.insert(items.end(), kv.second.begin(), kv.second.end());
-        (axis == 0 ? c.gx : c.gy).push_back(g);
-      }
-    }
-  }
+  for (size_t ci = 0; ci < bins.size(); ++ci) b->ch[ci].traj = bins[ci].second;
   auto bail = [&](int code) {
     gpe_batch_destroy(b);
     return code;
@@ -338,9 +399,7 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
   if (hipMalloc(&b->partials, (size_t)ntraj * gpe::SL_NSLOT * maxwg * sizeof(double)) != hipSuccess)
     return bail(GPE_ENOMEM);
   if (hipMalloc(&b->d_traj, ntraj * sizeof(gpe::TrajDev)) != hipSuccess) return bail(GPE_ENOMEM);
-  if (hipMalloc(&b->d_items, items.size() * sizeof(gpe::WorkItem)) != hipSuccess) return bail(GPE_ENOMEM);
-  if (hipMemcpy(b->d_items, items.data(), items.size() * sizeof(gpe::WorkItem), hipMemcpyHostToDevice) != hipSuccess)
-    return bail(GPE_EHIP);
+  if (build_items(b) != GPE_OK) return bail(GPE_ENOMEM);
   std::vector<double> kex(b->ktotal_x), key(b->ktotal_y);
   for (auto& t : b->tr) {
     for (int n : {t.nx, t.ny})
@@ -447,30 +506,7 @@ int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_gra
       const std::complex<double> e = std::exp(-(t.ky[k] * t.ky[k] / 2) * dt) * inv;
       ka[t.koff_y + k] = make_double2(e.real(), e.imag());
     }
-    gpe::TrajDev& d = b->h_traj[i];
-    d = gpe::TrajDev{};
-    d.psi = b->psi + t.off;
-    d.V = b->V + t.off;
-    d.kcol = b->kcol + t.koff_x;
-    d.kax = b->kax + t.koff_y;
-    d.keax_x = b->keax_x + t.koff_x;
-    d.keax_y = b->keax_y + t.koff_y;
-    d.tw_x = b->tw[t.nx];
-    d.tw_y = b->tw[t.ny];
-    d.st = b->st + i;
-    d.partials = b->partials + (size_t)i * gpe::SL_NSLOT * b->maxwg;
-    d.nx = (unsigned)t.nx;
-    d.ny = (unsigned)t.ny;
-    d.maxwg = b->maxwg;
-    d.tiles_x = t.tiles_x;
-    d.tiles_y = t.tiles_y;
-    d.renorm = (t.prm.gamma > 0 && t.prm.mu_init == 0) ? 1 : 0;
-    d.g = t.prm.g[0];
-    d.dr = dt.real();
-    d.di = dt.imag();
-    d.w = t.w;
-    d.NN0 = t.prm.Ntotal;
-    d.tau = t.prm.decay_rate;
+    b->h_traj[i] = traj_dev(b, i, dt);
   }
   BCHK(b, hipMemcpy(b->kcol, kc.data(), kc.size() * sizeof(double2), hipMemcpyHostToDevice));
   BCHK(b, hipMemcpy(b->kax, ka.data(), ka.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -533,6 +569,135 @@ int gpe_batch_split_outer(gpe_batch* b, int j, double* time2_out, double* muc, d
     if (ncur) ncur[i] = s[i].ncur;
   }
   return GPE_OK;
+}
+
+// The batch counterpart of gpe_tune: the (axis, length) pairs of the ensemble are
+// tuned one after another (coordinate descent), each by timing the whole ensemble's
+// steady-state step (x sweeps S_MID, then y sweeps S_KIN, one launch per length) with
+// that pair at each variant and the other pairs at their current choice.
+int gpe_batch_tune(gpe_batch* b, int remeasure, double* times_us, int cap, int* npairs) {
+  if (!b) return GPE_EINVAL;
+  BCHK(b, hipSetDevice(b->device));
+  std::vector<std::pair<int, int>> pairs;   // (axis, n), ascending
+  for (int a = 0; a < 2; ++a) {
+    std::map<int, int> seen;
+    for (auto& t : b->tr) seen[a == 0 ? t.nx : t.ny] = 1;
+    for (auto& kv : seen) pairs.push_back({a, kv.first});
+  }
+  if (npairs) *npairs = (int)pairs.size();
+  if (times_us)
+    for (int i = 0; i < std::min(cap, 2 * (int)pairs.size()); ++i) times_us[i] = std::nan("");
+  // current choice of every pair: recorded plan, else variant 0
+  std::map<std::pair<int, int>, int> cur;
+  std::vector<size_t> todo;
+  for (size_t pi = 0; pi < pairs.size(); ++pi) {
+    const int w = gpe::wisdom_get(2, pairs[pi].first, pairs[pi].second);
+    cur[pairs[pi]] = w > 0 && w < gpe::plan_variants(pairs[pi].second) ? w : 0;
+    if (gpe::plan_variants(pairs[pi].second) > 1 && (remeasure || w < 0)) todo.push_back(pi);
+  }
+  if (!todo.empty()) {
+    const int ntraj = (int)b->tr.size();
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < ntraj; ++i) {
+      const auto& t = b->tr[i];
+      const std::complex<double> dt = b->ddt != 0 ? cdiv({0.0, b->ddt}, {1.0, t.prm.gamma}) : std::complex<double>(0, 0);
+      b->h_traj[i] = traj_dev(b, i, dt);
+    }
+    BCHK(b, hipMemcpy(b->d_traj, b->h_traj.data(), ntraj * sizeof(gpe::TrajDev), hipMemcpyHostToDevice));
+    double2* save = nullptr;
+    gpe::WorkItem* items = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GPE_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+      if (e != hipSuccess && rc == GPE_OK) rc = bfail(b, GPE_EHIP, "batch tuning (%s): %s", what, hipGetErrorString(e));
+      return rc == GPE_OK;
+    };
+    size_t maxitems = 0;
+    for (auto& t : b->tr) {
+      const int cx = gpe::sweep_cols_min(t.nx), cy = gpe::sweep_cols_min(t.ny);
+      maxitems += (size_t)((t.ny + cx - 1) / cx + (t.nx + cy - 1) / cy);
+    }
+    // one ensemble step under the variants in `cur`: every x group, then every y group
+    struct TG { int axis, n, var; size_t first; int count; };
+    auto step_us = [&](double* us) {
+      std::vector<gpe::WorkItem> it;
+      std::vector<TG> groups;
+      for (auto& pr : pairs) {
+        const int axis = pr.first, n = pr.second, v = cur[pr];
+        TG g{axis, n, v, it.size(), 0};
+        for (int i = 0; i < ntraj; ++i) {
+          const auto& t = b->tr[i];
+          if ((axis == 0 ? t.nx : t.ny) != n) continue;
+          const int c = axis == 0 ? gpe::sweep_cols_per_block_phase(n, v) : gpe::sweep_cols_per_block(n, v);
+          const int tiles = ((axis == 0 ? t.ny : t.nx) + c - 1) / c;
+          for (int k = 0; k < tiles; ++k) it.push_back({i, k});
+        }
+        g.count = (int)(it.size() - g.first);
+        groups.push_back(g);
+      }
+      if (!chk(hipMemcpy(items, it.data(), it.size() * sizeof(gpe::WorkItem), hipMemcpyHostToDevice), "items"))
+        return;
+      const int warm = 2, reps = 4;
+      for (int r = 0; r < warm + reps && rc == GPE_OK; ++r) {
+        if (r == warm && !chk(hipEventRecord(e0, b->stream), "event")) return;
+        for (auto& g : groups) {
+          const gpe::BatchLaunch L{b->d_traj, items + g.first, g.count};
+          if (!chk(gpe::launch_batch_sweep(g.n, g.axis == 0, g.axis == 0 ? gpe::S_MID : gpe::S_KIN, g.var, L,
+                                           b->stream), "sweep"))
+            return;
+        }
+      }
+      float ms = 0.f;
+      if (chk(hipEventRecord(e1, b->stream), "event") && chk(hipEventSynchronize(e1), "sync") &&
+          chk(hipEventElapsedTime(&ms, e0, e1), "elapsed"))
+        *us = 1e3 * (double)ms / reps;
+    };
+    // the sweeps run in place: keep every trajectory's state and put it back afterwards
+    if (chk(hipMalloc(&save, b->total * sizeof(double2)), "save") &&
+        chk(hipMemcpyAsync(save, b->psi, b->total * sizeof(double2), hipMemcpyDeviceToDevice, b->stream), "save") &&
+        chk(hipMalloc(&items, maxitems * sizeof(gpe::WorkItem)), "items") && chk(hipEventCreate(&e0), "event") &&
+        chk(hipEventCreate(&e1), "event")) {
+      for (size_t pi : todo) {
+        if (rc) break;
+        // rounds alternate the variants; each keeps its best round
+        double t_us[2] = {1e300, 1e300};
+        for (int rv = 0; rv < 6 && rc == GPE_OK; ++rv) {
+          const int v = rv & 1;
+          cur[pairs[pi]] = v;
+          double us = 1e300;
+          step_us(&us);
+          t_us[v] = std::min(t_us[v], us);
+        }
+        if (rc) break;
+        if (times_us && 2 * (int)pi + 1 < cap) {
+          times_us[2 * pi] = t_us[0];
+          times_us[2 * pi + 1] = t_us[1];
+        }
+        // variant 0 is the default: the other plan must be 3 % faster to replace it
+        const int best = t_us[1] < 0.97 * t_us[0] ? 1 : 0;
+        cur[pairs[pi]] = best;
+        if (remeasure) gpe::wisdom_put(2, pairs[pi].first, pairs[pi].second, best);
+        else gpe::wisdom_put_if_absent(2, pairs[pi].first, pairs[pi].second, best);
+      }
+    }
+    if (save) {
+      chk(hipMemcpyAsync(b->psi, save, b->total * sizeof(double2), hipMemcpyDeviceToDevice, b->stream), "restore");
+      chk(hipStreamSynchronize(b->stream), "restore");
+    }
+    (void)hipFree(save);
+    (void)hipFree(items);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+  }
+  // adopt the recorded plans: new tiles and work items, no captured graph
+  for (auto& pr : pairs) {
+    const int w = gpe::wisdom_get(2, pr.first, pr.second);
+    (pr.first == 0 ? b->var_x : b->var_y)[pr.second] = w > 0 && w < gpe::plan_variants(pr.second) ? w : 0;
+  }
+  drop_graph(b);
+  b->begun = false;
+  return build_items(b);
 }
 
 }  // extern "C"

@@ -14,9 +14,9 @@
 
 namespace gpe {
 
-template <int N, bool COLFAST, int SEQ>
+template <int N, bool COLFAST, int SEQ, int VAR>
 __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
-  constexpr int C = sweep_cols<N, SEQ>();
+  constexpr int C = sweep_cols<N, SEQ, VAR>();
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN ? 2 * N : N];
   __shared__ double red[4];
@@ -53,19 +53,29 @@ __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
     p.stride = 1;
     p.slot_ke = SL_KY;
   }
-  sweep_body<N, C, COLFAST, SEQ>(p, (unsigned)it.tile, (unsigned)it.tile, lds, twl, red);
+  sweep_body<N, C, COLFAST, SEQ, VAR>(p, (unsigned)it.tile, (unsigned)it.tile, lds, twl, red);
 }
 
 #define GPE_CAT2(a, b) a##b
 #define GPE_CAT(a, b) GPE_CAT2(a, b)
 
-// items must all belong to trajectories whose swept axis has length GPE_N
-hipError_t GPE_CAT(launch_batch_sweep_, GPE_N)(bool xaxis, int seq, const BatchLaunch& b, hipStream_t s) {
+// items must all belong to trajectories whose swept axis has length GPE_N, with tiles
+// of the plan variant `var`
+hipError_t GPE_CAT(launch_batch_sweep_, GPE_N)(bool xaxis, int seq, int var, const BatchLaunch& b, hipStream_t s) {
   constexpr int N = GPE_N;
   if (b.nitems <= 0) return hipSuccess;
   const dim3 grid((unsigned)b.nitems), block(256);
+#if GPE_N == 64 || GPE_N == 128 || GPE_N == 256 || GPE_N == 384 || GPE_N == 512
+#define GPE_L(CF, S)                                                                              \
+  case S:                                                                                         \
+    if (var == 1) hipLaunchKernelGGL((batch_sweep_kernel<N, CF, S, 1>), grid, block, 0, s, b);    \
+    else hipLaunchKernelGGL((batch_sweep_kernel<N, CF, S, 0>), grid, block, 0, s, b);              \
+    break;
+#else
+  if (var != 0) return hipErrorInvalidValue;
 #define GPE_L(CF, S) \
-  case S: hipLaunchKernelGGL((batch_sweep_kernel<N, CF, S>), grid, block, 0, s, b); break;
+  case S: hipLaunchKernelGGL((batch_sweep_kernel<N, CF, S, 0>), grid, block, 0, s, b); break;
+#endif
   if (xaxis) {
     switch (seq) {
       GPE_L(true, S_OPEN) GPE_L(true, S_MID) GPE_L(true, S_CLOSE) GPE_L(true, S_KEX)

@@ -198,6 +198,26 @@ struct Plan256x8 {
   __device__ __host__ static constexpr int ns(int p) { return p == 0 ? 1 : p == 1 ? 8 : p == 2 ? 64 : 256; }
 };
 
+// Alternative plans measured by the plan tuner (gpe_tune): fewer elements per thread
+// (more threads per pencil, fewer pencils per work-group) for 64, 128 and 512, more
+// for 384.  Each radix divides E.
+#define GPE_PLAN_NAMED(NAME, N_, E_, P_, A_, B_, C_)                               \
+  struct NAME {                                                                    \
+    static constexpr int N = N_, E = E_, T = N_ / E_, P = P_;                      \
+    static constexpr int R0 = A_, R1 = B_, R2 = C_, R3 = 1;                        \
+    __device__ __host__ static constexpr int radix(int p) {                        \
+      return p == 0 ? A_ : p == 1 ? B_ : p == 2 ? C_ : 1;                          \
+    }                                                                              \
+    __device__ __host__ static constexpr int ns(int p) {                           \
+      return p == 0 ? 1 : p == 1 ? A_ : p == 2 ? A_ * B_ : A_ * B_ * C_;            \
+    }                                                                              \
+  };
+GPE_PLAN_NAMED(Plan64x8, 64, 8, 2, 8, 8, 1)
+GPE_PLAN_NAMED(Plan128x8, 128, 8, 3, 8, 4, 4)
+GPE_PLAN_NAMED(Plan384x24, 384, 24, 3, 12, 8, 4)
+GPE_PLAN_NAMED(Plan512x8, 512, 8, 3, 8, 8, 8)
+#undef GPE_PLAN_NAMED
+
 // One Stockham pass on the canonical layout.  Butterfly j = t + T*b takes
 // elements j + r*N/R (register b + r*B), twiddles them by W_{NS R}^{r (j mod NS)}
 // and applies a radix-R DFT; output r of butterfly j belongs at

@@ -77,12 +77,17 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   // z pencils: columns (iy, ix), base = col, stride nx*ny
   h->ax[2].stride = nx * ny; h->ax[2].ncols = nx * ny; h->ax[2].A = nx * ny; h->ax[2].sa = 1; h->ax[2].sb = 0;
 
+  // plans recorded in the wisdom for this grid's axes (gpe_tune measures missing ones)
+  for (int a = 0; a < h->ndims; ++a) {
+    const int v = gpe::wisdom_get(h->ndims, a, nn[a]);
+    h->var[a] = v > 0 && v < gpe::plan_variants(nn[a]) ? v : 0;
+  }
   int maxwg = 2048;
-  for (int a = 0; a < h->ndims; ++a)
-    for (int C : {gpe::sweep_cols_per_block(nn[a]), gpe::sweep_cols_per_block_phase(nn[a])}) {
-      const long long nb = (h->ax[a].ncols + C - 1) / C;
-      if (nb > maxwg) maxwg = (int)nb;
-    }
+  for (int a = 0; a < h->ndims; ++a) {
+    const int C = gpe::sweep_cols_min(nn[a]);  // partials fit every variant
+    const long long nb = (h->ax[a].ncols + C - 1) / C;
+    if (nb > maxwg) maxwg = (int)nb;
+  }
   {  // two-component x-sweeps put C/2 columns in a work-group (one lane per component)
     const int C2 = gpe::sweep_cols_per_block(nn[0]) / 2;
     const long long nb = (h->ax[0].ncols + C2 - 1) / C2;

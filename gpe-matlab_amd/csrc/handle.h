@@ -118,6 +118,9 @@ struct gpe_handle {
   // vortex-core detection (gpe_detect_core*): one grown scratch allocation
   char* dc_buf = nullptr;
   size_t dc_bytes = 0;
+  // FFT plan variant of the tunable sweeps along each axis, fixed when the handle is
+  // created (from the plan wisdom) or tuned (gpe_tune)
+  int var[3] = {0, 0, 0};
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // one outer iteration of gpe_split_outer captured as a graph (static potential,
@@ -258,7 +261,7 @@ inline int sweep(gpe_handle* h, int axis, int seq, const SweepParams& p, int* nb
     e1 = prof_event(h);
     if (e0) hipEventRecord(e0, h->stream);
   }
-  hipError_t e = gpe::launch_sweep(h->ax[axis].n, seq, colfast, p, h->stream, nb);
+  hipError_t e = gpe::launch_sweep(h->ax[axis].n, seq, colfast, p, h->stream, nb, h->var[axis]);
   if (h->prof && e0 && e1) {
     hipEventRecord(e1, h->stream);
     h->prof_recs.push_back({seq * 3 + axis, {e0, e1}});

@@ -38,9 +38,19 @@ hipError_t launch_itp_check(const double2* psi, double2* old, const DevState* st
                             int maxwg, int nblocks, hipStream_t s);
 hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int nblocks, hipStream_t s,
                         bool reset = true);
-int sweep_cols_per_block(int n);
+// pencils per work-group of the non-phase sweeps of length n under plan variant var
+int sweep_cols_per_block(int n, int var = 0);
 // the same for the phase sweeps (open / mid / close / kex / ITP on the x axis)
-int sweep_cols_per_block_phase(int n);
+int sweep_cols_per_block_phase(int n, int var = 0);
+// fewest pencils per work-group over every variant and sequence class (sizes partials)
+int sweep_cols_min(int n);
+// number of plan variants of length n (0: unsupported length)
+int plan_variants(int n);
+// process-wide plan wisdom keyed by (dimensions, axis, length); -1 = none recorded
+int wisdom_get(int ndims, int axis, int n);
+void wisdom_put(int ndims, int axis, int n, int var);
+bool wisdom_put_if_absent(int ndims, int axis, int n, int var);
+void wisdom_clear();
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
 
 }  // namespace gpe

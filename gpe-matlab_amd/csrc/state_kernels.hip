@@ -1,5 +1,10 @@
 // Block-boundary kernels: second-stage reductions, device-side scalar updates
 // (norm, renormalisation, chemical potential) and small elementwise passes.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "fft_dev.h"
 #include "phase_dev.h"
 #include "reduce_dev.h"
@@ -76,7 +81,7 @@ hipError_t launch_scale(double2* psi, DevState* st, int which, long long n, int 
 }
 
 // ------------------------------------------------------------ length dispatch
-#define GPE_DECL(N) hipError_t launch_sweep_##N(int, bool, const SweepParams&, hipStream_t, int*);
+#define GPE_DECL(N) hipError_t launch_sweep_##N(int, bool, int, const SweepParams&, hipStream_t, int*);
 GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
 GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
 #undef GPE_DECL
@@ -91,11 +96,29 @@ bool sweep_length_supported(int n) {
   }
 }
 
-int sweep_cols_per_block_phase(int n) {
-  return n == 256 ? sweep_cols<256, S_MID>() : sweep_cols_per_block(n);
+int plan_variants(int n) {
+  switch (n) {
+    case 64: case 128: case 256: case 384: case 512: return 2;
+    default: return sweep_length_supported(n) ? 1 : 0;
+  }
 }
 
-int sweep_cols_per_block(int n) {
+int sweep_cols_per_block_phase(int n, int var) {
+  if (n == 256) return var ? sweep_cols<256, S_MID, 1>() : sweep_cols<256, S_MID, 0>();
+  return sweep_cols_per_block(n, var);
+}
+
+int sweep_cols_per_block(int n, int var) {
+  if (var == 1 && plan_variants(n) == 2) {
+    switch (n) {
+      case 64: return sweep_cols<64, S_KIN, 1>();
+      case 128: return sweep_cols<128, S_KIN, 1>();
+      case 256: return sweep_cols<256, S_KIN, 1>();
+      case 384: return sweep_cols<384, S_KIN, 1>();
+      case 512: return sweep_cols<512, S_KIN, 1>();
+      default: break;
+    }
+  }
   switch (n) {
     case 8: return 256 / Plan<8>::T;
     case 12: return 256 / Plan<12>::T;
@@ -114,6 +137,40 @@ int sweep_cols_per_block(int n) {
   }
 }
 
+int sweep_cols_min(int n) {
+  int c = 1 << 30;
+  for (int v = 0; v < plan_variants(n); ++v)
+    c = std::min(c, std::min(sweep_cols_per_block(n, v), sweep_cols_per_block_phase(n, v)));
+  return c;
+}
+
+// Process-wide plan wisdom: the measured variant per (dimensions, axis, length).
+namespace {
+std::mutex g_wisdom_mu;
+std::map<std::tuple<int, int, int>, int> g_wisdom;
+}  // namespace
+
+int wisdom_get(int ndims, int axis, int n) {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  auto it = g_wisdom.find({ndims, axis, n});
+  return it == g_wisdom.end() ? -1 : it->second;
+}
+
+void wisdom_put(int ndims, int axis, int n, int var) {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  g_wisdom[{ndims, axis, n}] = var;
+}
+
+bool wisdom_put_if_absent(int ndims, int axis, int n, int var) {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  return g_wisdom.emplace(std::make_tuple(ndims, axis, n), var).second;
+}
+
+void wisdom_clear() {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  g_wisdom.clear();
+}
+
 #define GPE_DECL(N) hipError_t launch_spinor_sweep_##N(int, const SpinorParams&, hipStream_t, int*);
 GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
 GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
@@ -130,9 +187,9 @@ hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_
   }
 }
 
-hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks) {
+hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks, int var) {
   switch (n) {
-#define GPE_C(N) case N: return launch_sweep_##N(seq, colfast, p, s, nblocks);
+#define GPE_C(N) case N: return launch_sweep_##N(seq, colfast, var, p, s, nblocks);
     GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
     GPE_C(256) GPE_C(384) GPE_C(512)
 #undef GPE_C

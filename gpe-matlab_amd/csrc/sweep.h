@@ -138,7 +138,9 @@ struct SpinorParams {
 };
 
 // Launch one sweep; returns hipErrorInvalidValue for an unsupported length.
-hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks);
+// var selects the plan variant of the tunable sequences (0 = default plan).
+hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks,
+                        int var = 0);
 bool sweep_length_supported(int n);
 hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks);
 

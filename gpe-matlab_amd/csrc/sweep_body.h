@@ -23,12 +23,30 @@ constexpr bool phase_seq(int s) {
          s == S_SMID;
 }
 
-// FFT plan of a sweep: Plan<N>, except 8.8.4 for the phase sweeps of 256-point pencils.
-template <int N, int SEQ> struct SweepPlan { using type = Plan<N>; };
-template <int SEQ> struct SweepPlan<256, SEQ> {
-  using type = typename std::conditional<phase_seq(SEQ), Plan256x8, Plan<256>>::type;
+// Sequences whose plan the tuner may switch (the solve_split and measurement sweeps);
+// every other sequence always runs variant 0.
+constexpr bool tunable_seq(int s) {
+  return s == S_FWD || s == S_INV || s == S_KIN || s == S_KE || s == S_KEX || s == S_OPEN || s == S_MID ||
+         s == S_CLOSE || s == S_CLOSEM;
+}
+
+// Second plan of a length (variant 1), where one exists.
+template <int N> struct AltPlan { static constexpr bool has = false; using type = Plan<N>; };
+template <> struct AltPlan<64> { static constexpr bool has = true; using type = Plan64x8; };
+template <> struct AltPlan<128> { static constexpr bool has = true; using type = Plan128x8; };
+template <> struct AltPlan<384> { static constexpr bool has = true; using type = Plan384x24; };
+template <> struct AltPlan<512> { static constexpr bool has = true; using type = Plan512x8; };
+template <int N> constexpr bool has_variant() { return N == 256 || AltPlan<N>::has; }
+
+// FFT plan of a sweep.  Variant 0: Plan<N>, except 8.8.4 for the phase sweeps of
+// 256-point pencils.  Variant 1: AltPlan<N>; for 256 the other of 16.16 and 8.8.4.
+template <int N, int SEQ, int VAR = 0> struct SweepPlan {
+  using type = typename std::conditional<VAR == 0, Plan<N>, typename AltPlan<N>::type>::type;
 };
-template <int N, int SEQ> constexpr int sweep_cols() { return 256 / SweepPlan<N, SEQ>::type::T; }
+template <int SEQ, int VAR> struct SweepPlan<256, SEQ, VAR> {
+  using type = typename std::conditional<phase_seq(SEQ) != (VAR == 1), Plan256x8, Plan<256>>::type;
+};
+template <int N, int SEQ, int VAR = 0> constexpr int sweep_cols() { return 256 / SweepPlan<N, SEQ, VAR>::type::T; }
 
 template <int SEQ> struct SeqTraits {
   static constexpr bool LOAD_V = SEQ == S_OPEN || SEQ == S_MID || SEQ == S_CLOSE || SEQ == S_CLOSEM ||
@@ -60,10 +78,10 @@ __device__ __forceinline__ double ke_weight(const SweepParams& p, int i, double 
 // holds Lds<N, C, COLFAST>::SIZE doubles, `twl` 2N entries (twiddles, then the
 // S_KIN per-index kinetic factors), `red` 4 doubles;
 // reductions go to partials[slot * maxwg + wg].
-template <int N, int C, bool COLFAST, int SEQ>
+template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
 __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, unsigned wg, double* lds,
                                            double2* twl, double* red) {
-  using PL = typename SweepPlan<N, SEQ>::type;
+  using PL = typename SweepPlan<N, SEQ, VAR>::type;
   using TR = SeqTraits<SEQ>;
   constexpr int T = PL::T, E = PL::E;
   static_assert(C * T == 256, "work-group is 256 threads");
@@ -312,7 +330,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
 #define GPE_WAVES_PHASE 1
 #endif
 
-template <int N, int C, bool COLFAST, int SEQ>
+template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     SEQ == S_KIN ? GPE_WAVES_KIN
                  : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE || SEQ == S_CLOSEM) ? GPE_WAVES_PHASE : 1)))
@@ -322,7 +340,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   __shared__ double red[4];
   const unsigned b = C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const unsigned tile = p.reverse ? gridDim.x - 1u - b : b;
-  sweep_body<N, C, COLFAST, SEQ>(p, tile, tile, lds, twl, red);
+  sweep_body<N, C, COLFAST, SEQ, VAR>(p, tile, tile, lds, twl, red);
 }
 
 }  // namespace gpe

@@ -97,6 +97,12 @@ SIGNATURES = {
     "gpe_batch_get_state": [_P, C.c_int, _DP],
     "gpe_batch_split_begin": [_P, _D, C.c_int, C.c_int, _DP],
     "gpe_batch_split_outer": [_P, C.c_int, _DP, _DP, _DP],
+    "gpe_tune": [_P, C.c_int, _DP],
+    "gpe_get_plan": [_P, C.c_int, _IP],
+    "gpe_batch_tune": [_P, C.c_int, _DP, C.c_int, _IP],
+    "gpe_wisdom_get": [C.c_int, C.c_int, C.c_int],
+    "gpe_wisdom_set": [C.c_int, C.c_int, C.c_int, C.c_int],
+    "gpe_wisdom_forget": [],
 }
 OPTIONAL = set()
 
@@ -176,6 +182,7 @@ class Device:
             raise GPELibraryError(f"gpe_create failed ({ERRNAMES.get(rc, rc)}) for grid n={n}")
         self.h = h
         self.shape = grid.size
+        self.ndims = grid.ndims
         self.npts = int(np.prod(grid.size))
 
     def close(self):
@@ -191,6 +198,23 @@ class Device:
 
     def call(self, name, *args):
         check(getattr(self.lib, name)(self.h, *args), self.h, name)
+
+    def tune(self, remeasure=False):
+        """Measure the FFT plan variants of this grid (GPEtask.m:55-58); returns the
+        average launch times in microseconds as a (3, 2) array [axis, variant]
+        (NaN where the wisdom already held a plan or the length has one plan)."""
+        t = np.full(6, np.nan)
+        self.call("gpe_tune", int(bool(remeasure)), dptr(t))
+        return t.reshape(3, 2)
+
+    def plans(self):
+        """Plan variant of each axis held by this handle."""
+        v = C.c_int()
+        out = []
+        for a in range(self.ndims):
+            self.call("gpe_get_plan", a, C.byref(v))
+            out.append(v.value)
+        return out
 
     def set_params(self, ncomp, g, gamma, decay_rate, Ntotal, mu_init, n_recalc, nlincpl=0.0):
         p = Params()
@@ -422,6 +446,7 @@ class Batch:
             raise GPELibraryError(f"gpe_batch_create failed ({ERRNAMES.get(rc, rc)})")
         self.h = h
         self.shapes = [g.size for g in grids]
+        self.lengths = [(g.nx, g.ny) for g in grids]
         self.n = n
 
     def close(self):
@@ -451,6 +476,17 @@ class Batch:
         self.call("gpe_batch_get_state", int(i), out.view(np.float64).ctypes.data_as(_DP))
         return from_colmajor_c128(out, self.shapes[i])
 
+    def tune(self, remeasure=False):
+        """Measure the plan variants of every (axis, length) of the ensemble; returns
+        {(axis, n): (t_variant0_us, t_variant1_us)} for the measured pairs."""
+        cap = 64
+        t = np.full(cap, np.nan)
+        npairs = C.c_int()
+        self.call("gpe_batch_tune", int(bool(remeasure)), dptr(t), cap, C.byref(npairs))
+        keys = sorted({(0, nx) for nx, _ in self.lengths} | {(1, ny) for _, ny in self.lengths})
+        return {k: (t[2 * i], t[2 * i + 1]) for i, k in enumerate(keys[:npairs.value])
+                if 2 * i + 1 < cap and not np.isnan(t[2 * i])}
+
     def split_begin(self, ddt, niter_inner, use_graph=True):
         muc = np.zeros(self.n)
         self.call("gpe_batch_split_begin", float(ddt), int(niter_inner), int(bool(use_graph)), dptr(muc))
@@ -462,6 +498,22 @@ class Batch:
         ncur = np.zeros(self.n)
         self.call("gpe_batch_split_outer", int(j), C.byref(t), dptr(muc), dptr(ncur))
         return t.value, muc, ncur
+
+
+def wisdom_get(ndims, axis, n):
+    """Plan variant recorded for (ndims, axis, length), or -1."""
+    return int(load().gpe_wisdom_get(int(ndims), int(axis), int(n)))
+
+
+def wisdom_set(ndims, axis, n, variant):
+    rc = load().gpe_wisdom_set(int(ndims), int(axis), int(n), int(variant))
+    if rc != GPE_OK:
+        raise GPELibraryError(f"gpe_wisdom_set({ndims}, {axis}, {n}, {variant}): {ERRNAMES.get(rc, rc)}")
+
+
+def wisdom_forget():
+    """fftw('dwisdom', []): forget every recorded plan."""
+    load().gpe_wisdom_forget()
 
 
 def selftest_phase(er, ei):

@@ -14,12 +14,15 @@ from . import _lib
 
 
 class GPEEnsemble:
-    def __init__(self, tasks, use_graph=True):
+    def __init__(self, tasks, use_graph=True, tune=True):
         if not tasks:
             raise ValueError("empty ensemble")
         self.tasks = list(tasks)
         self.use_graph = use_graph
         self._b = _lib.Batch([t.grid for t in self.tasks])
+        # FFT plans of the ensemble's lengths, measured on its own sweeps where the
+        # process's plan wisdom holds none (the counterpart of GPEtask.m:55-58)
+        self.tune_times = self._b.tune() if tune else {}
 
     def solve_split(self, ddt, niter_inner, niter_outer):
         """solve_split.m:1-84 for every task; returns the list of final states."""

@@ -50,6 +50,9 @@ class GPEtask:
         self.itp_min_iter = 50
         self.itp_adjust_stepsize = False
         self.show_image = 0
+        # GPEtask.m:28, 55-58: measure the FFT plans of this grid when the task first
+        # reaches the device (plans already in the process's wisdom are reused)
+        self.fftw_set_wisdom = True
         if callable(trappot):
             self.Vtrap = trappot(grid.mesh.x, grid.mesh.y, grid.mesh.z)
         else:
@@ -66,6 +69,8 @@ class GPEtask:
     def _device(self):
         if self._dev is None:
             self._dev = _lib.Device(self.grid)
+            if self.fftw_set_wisdom:
+                self._dev.tune()
         return self._dev
 
     @property

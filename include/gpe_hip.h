@@ -270,6 +270,35 @@ int gpe_batch_split_begin(gpe_batch* b, double ddt, int niter_inner, int use_gra
 /* One outer iteration j for every trajectory; muc[ntraj], ncur[ntraj] as passed to ext_callback. */
 int gpe_batch_split_outer(gpe_batch* b, int j, double* time2, double* muc, double* ncur);
 
+/* ------------------------------------------------------------------ plan tuning
+ * Replaces the FFTW planning the GPEtask constructor does when fftw_set_wisdom is
+ * true (lib/@GPEtask/GPEtask.m:28 and 55-58: fftw('dwisdom',[]);
+ * fftw('planner','measure'); fftn(grid.mesh.x)).  Pencil lengths 64, 128, 256, 384
+ * and 512 have two FFT plans (variant 0 and 1, with different radices, elements per
+ * thread and pencils per work-group).  The wisdom is a process-wide table keyed by
+ * (ndims, axis, n); gpe_create and gpe_batch_create adopt the variants it holds
+ * (variant 0 where it holds none).
+ *
+ * gpe_tune times the steady-state solve_split step of the handle's grid (x phase
+ * sweep; 3D: y forward, z kinetic, y inverse; 2D: y kinetic) for every combination
+ * of variants of the axes the wisdom has no plan for (all axes with remeasure != 0),
+ * records the fastest combination and adopts the recorded variants.  The state is
+ * preserved.  times_us[2*axis + variant] receives the best step time in microseconds
+ * among the combinations with that axis at that variant (NaN where nothing was
+ * measured); it may be null.  gpe_batch_tune tunes every (axis, length) pair of an
+ * ensemble in turn, timing the whole ensemble's step with the pair at each variant
+ * (times_us, if not null, holds 2 entries per pair in ascending (axis, length) order,
+ * up to cap entries; *npairs receives the number of pairs); call it before
+ * gpe_batch_split_begin. */
+int gpe_tune(gpe_handle* h, int remeasure, double* times_us);
+int gpe_get_plan(gpe_handle* h, int axis, int* variant);
+int gpe_batch_tune(gpe_batch* b, int remeasure, double* times_us, int cap, int* npairs);
+/* Plan variant recorded for (ndims, axis, n), or -1 if none. */
+int gpe_wisdom_get(int ndims, int axis, int n);
+int gpe_wisdom_set(int ndims, int axis, int n, int variant);
+/* fftw('dwisdom', []): forget every recorded plan (handles keep the plans they hold). */
+int gpe_wisdom_forget(void);
+
 /* Measurement hooks (no reference counterpart): time every sweep launch with
  * HIP events recorded on the handle's own stream.  gpe_profile_read fills
  * ms[seq*3 + axis] (summed milliseconds) and count[seq*3 + axis] for
