@@ -36,8 +36,10 @@ PEAK_HBM_GBPS = 8000.0
 # algorithmic HBM bytes per grid point per launch of each sweep sequence
 SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40, "kex": 24, "ke": 16,
              "itpa": 40, "itpb": 48, "closem": 40, "rkin": 32, "ropen": 40, "rmid": 40, "rclosem": 40,
-             "ritpa": 40, "ritpb": 48, "sopen": 40, "smid": 40, "kinm": 32}
+             "ritpa": 40, "ritpb": 48, "sopen": 40, "smid": 40, "kinm": 32,
+             "ykin": 32, "yopen": 40, "ymid": 40, "yclosem": 40}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
+STEP_BYTES_3D_YSPLIT = 72  # two-sweep step: 2 x 32 B + 8 B of V
 
 
 def committed_traffic(n):
@@ -74,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
+    ap.add_argument("--ysplit", action="store_true",
+                    help="c2: run the two-sweep step (256-point y transform split 16 x 16, GPE_YSPLIT=1)")
     ap.add_argument("--omega", type=float, default=0.0,
                     help="c2: rotation frequency (rotating-frame kinetic step); reported as a variant")
     ap.add_argument("--copy-calib", action="store_true",
@@ -82,6 +86,8 @@ def main():
                     help="c2 (default): 3D 256^3 solve_split; c1: 2D ITP + vortex dynamics; c3: adjoint "
                          "gradient cost; c4: ragged ensemble; c5: 3D spinor 192^3")
     args = ap.parse_args()
+    if args.ysplit:
+        os.environ["GPE_YSPLIT"] = "1"    # read when the handle is created
 
     if args.copy_calib:
         import torch
@@ -221,6 +227,7 @@ def main():
                 "traffic_source": "profiles/sweep256_c2_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
                 if traffic else None}
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
+    ysplit_on = any(SEQ_NAMES[seq] == "ymid" for (seq, _ax) in prof)
 
     out = {
         "metric": "split-steps/s (GPEtask.solve_split, 256^3 complex128)",
@@ -245,6 +252,9 @@ def main():
                       "tune_us": [[None if not np.isfinite(x) else round(float(x), 1) for x in r] for r in tune_us]},
         "step_alg_GBps": round(step_gbps, 1),
         "step_roofline_frac": round(step_gbps / PEAK_HBM_GBPS, 4),
+        "schedule": ("two sweeps per step, y transform split 16 x 16 (72 B/pt moved; step_alg_GBps counts "
+                     "the 136 B/pt four-sweep schedule)") if ysplit_on else "four sweeps per step (136 B/pt)",
+        "step_moved_GBps": round((STEP_BYTES_3D_YSPLIT if ysplit_on else STEP_BYTES_3D) * npts * steps_per_s / 1e9, 1),
         "parity_rel_l2_vs_oracle": parity,
         "parity_steps": args.cpu_steps if parity is not None else 0,
         "parity_curve": parity_curve,

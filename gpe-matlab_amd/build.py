@@ -61,6 +61,8 @@ def build(jobs=None, verbose=True, out=None, build_dir=None):
         tasks.append(("spinor_kernels.hip", os.path.join(BUILD, f"spinor_{n}.o"), (f"-DGPE_N={n}",)))
         tasks.append(("adjoint_kernels.hip", os.path.join(BUILD, f"adjoint_{n}.o"), (f"-DGPE_N={n}",)))
         tasks.append(("batch_kernels.hip", os.path.join(BUILD, f"batch_{n}.o"), (f"-DGPE_N={n}",)))
+    for n in (64, 128, 256):   # x lengths of the y-split 3D step
+        tasks.append(("ysplit_kernels.hip", os.path.join(BUILD, f"ysplit_{n}.o"), (f"-DGPE_N={n}",)))
     for s in SOURCES:
         if os.path.exists(os.path.join(CSRC, s)):
             tasks.append((s, os.path.join(BUILD, s.replace(".hip", ".o")), ()))

@@ -171,6 +171,7 @@ int build_v(gpe_handle* h, const double* coeffs, int m) {
                      h->adj_G, c, h->adj_nbasis, (long long)h->npts);
   HIPCHK(h, hipGetLastError());
   h->have_V[0] = true;
+  ++h->v_gen;
   return GPE_OK;
 }
 

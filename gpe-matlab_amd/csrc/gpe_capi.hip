@@ -47,6 +47,12 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
   h->snake = snake ? atoi(snake) : 0;
   const char* slabs = getenv("GPE_SLABS");
   h->slabs = slabs ? atoi(slabs) : 0;
+  const char* ys = getenv("GPE_YSPLIT");
+  h->ysplit_off = !(ys && atoi(ys) == 1);   // the two-sweep step is opt-in (GPE_YSPLIT=1)
+  const char* yna = getenv("GPE_YSPLIT_NA");
+  if (yna && (atoi(yna) == 1 || atoi(yna) == 2)) h->ysplit_na = atoi(yna);
+  const char* ypv = getenv("GPE_YSPLIT_PLAN");
+  if (ypv && (atoi(ypv) == 0 || atoi(ypv) == 1)) h->ysplit_pv = atoi(ypv);
   h->ndims = d->ndims;
   h->nx = d->n[0];
   h->ny = d->n[1];
@@ -154,6 +160,9 @@ int gpe_destroy(gpe_handle* h) {
   }
   (void)hipFree(h->rtab_x);
   (void)hipFree(h->rtab_y);
+  (void)hipFree(h->Vp);
+  (void)hipFree(h->kcolB);
+  (void)hipFree(h->twy);
   (void)hipFree(h->cpl);
   (void)hipFree(h->kcol);
   (void)hipFree(h->kax);
@@ -230,6 +239,7 @@ int gpe_set_potential(gpe_handle* h, int comp, const double* V) {
   if (!h || !V || comp < 0 || comp > 1 || !h->V[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
   HIPCHK(h, hipSetDevice(h->device));
   HIPCHK(h, hipMemcpyAsync(h->V[comp], V, h->npts * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (comp == 0) ++h->v_gen;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_V[comp] = true;
   return GPE_OK;
@@ -344,6 +354,7 @@ int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out)
     return gpe::spinor_split_begin(h, ddt, niter_inner, muc_out);
   }
   if ((rc = materialize(h))) return rc;
+  if ((rc = ysplit_prepare(h))) return rc;
   double m[5];
   if ((rc = gpe_measure(h, m))) return rc;
   // solve_split.m:17-21: NN0 = Ntotal, or the norm of the state
@@ -543,6 +554,26 @@ int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n) {
   }
   h->prof_recs.clear();
   return GPE_OK;
+}
+
+int gpe_selftest_rowdft(const double* in, double* out_f, double* out_i) {
+  if (!in || !out_f || !out_i) return GPE_EINVAL;
+  double2 *din = nullptr, *df = nullptr, *di = nullptr, *dtw = nullptr;
+  std::vector<double2> tw = twiddles(256);
+  int rc = GPE_EHIP;
+  if (hipMalloc(&din, 128 * sizeof(double2)) == hipSuccess && hipMalloc(&df, 128 * sizeof(double2)) == hipSuccess &&
+      hipMalloc(&di, 128 * sizeof(double2)) == hipSuccess && hipMalloc(&dtw, 256 * sizeof(double2)) == hipSuccess &&
+      hipMemcpy(din, in, 128 * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dtw, tw.data(), 256 * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+      gpe::launch_rowdft_selftest(din, df, di, dtw, nullptr) == hipSuccess &&
+      hipMemcpy(out_f, df, 128 * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess &&
+      hipMemcpy(out_i, di, 128 * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess)
+    rc = GPE_OK;
+  (void)hipFree(din);
+  (void)hipFree(df);
+  (void)hipFree(di);
+  (void)hipFree(dtw);
+  return rc;
 }
 
 int gpe_selftest_phase(int n, const double* er, const double* ei, double* out) {

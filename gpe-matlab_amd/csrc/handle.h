@@ -121,6 +121,15 @@ struct gpe_handle {
   // FFT plan variant of the tunable sweeps along each axis, fixed when the handle is
   // created (from the plan wisdom) or tuned (gpe_tune)
   int var[3] = {0, 0, 0};
+  // y-split 3D step (ny = 256): potential in the x sweeps' tile order, kinetic factor
+  // per z pencil in the split's spectral order; v_gen counts writes of V[0]
+  bool ysplit_off = true;       // on with GPE_YSPLIT=1
+  int ysplit_na = 1;            // values of a per x-sweep tile (GPE_YSPLIT_NA)
+  int ysplit_pv = 1;            // plan of the x sweeps (GPE_YSPLIT_PLAN)
+  double* Vp = nullptr;
+  double2* kcolB = nullptr;
+  double2* twy = nullptr;       // 256-point twiddles of the y axis
+  long long v_gen = 0, vp_gen = -1;
   int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
   int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // one outer iteration of gpe_split_outer captured as a graph (static potential,
@@ -275,7 +284,7 @@ inline int mid_axis(const gpe_handle* h) { return h->ndims == 3 ? 2 : 1; }
 // Kinetic tables for complex dt: kcol[col] * kax[i] = exp(-kk dt) / npts
 // (set_kinop.m:9) split into the mid-spectrum axis and its columns.
 inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, double2* kax, double frac = 1.0,
-                         double frac2 = 0.0) {
+                         double frac2 = 0.0, double2* kcolB = nullptr) {
   const int ma = mid_axis(h);
   const Axis& m = h->ax[ma];
   std::vector<double2> col, axv(m.n);
@@ -307,6 +316,19 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
   }
   HIPCHK(h, hipMemcpyAsync(kcol, col.data(), col.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(kax, axv.data(), axv.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  std::vector<double2> colB;
+  if (kcolB && h->ndims == 3 && h->ny == 256) {
+    // y-split step: the z pencil at y position l + 16 s holds ky index
+    // bitrev4(s) + 16 bitrev4(l) (16 x 16 split, both factors in bit-reversed order)
+    auto br4 = [](int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); };
+    colB.resize(col.size());
+    for (int ix = 0; ix < h->nx; ++ix)
+      for (int pos = 0; pos < 256; ++pos) {
+        const int ky = br4(pos >> 4) + 16 * br4(pos & 15);
+        colB[(size_t)pos + 256 * (size_t)ix] = col[(size_t)ky + 256 * (size_t)ix];
+      }
+    HIPCHK(h, hipMemcpyAsync(kcolB, colB.data(), colB.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return GPE_OK;
 }
@@ -360,8 +382,33 @@ inline int build_kinetic_rot(gpe_handle* h, std::complex<double> dt) {
 }
 
 // The kinetic tables of the current frame.
+// The y-split 3D step can run on this handle (its tables are set up by ysplit_prepare).
+inline bool ysplit_eligible(const gpe_handle* h) {
+  return !h->ysplit_off && h->ndims == 3 && h->ny == 256 && h->omega == 0.0 && h->prm.ncomp == 1 &&
+         h->slabs <= 1 && gpe::ysplit_x_supported(h->nx) && gpe::ysplit_x_supported(h->nz) && h->nz % 8 == 0;
+}
+
 inline int build_step_tables(gpe_handle* h, std::complex<double> dt) {
-  return h->omega != 0.0 ? build_kinetic_rot(h, dt) : build_kinetic(h, dt, h->kcol, h->kax);
+  if (h->omega != 0.0) return build_kinetic_rot(h, dt);
+  return build_kinetic(h, dt, h->kcol, h->kax, 1.0, 0.0, ysplit_eligible(h) ? h->kcolB : nullptr);
+}
+
+// Buffers of the y-split step (allocated once) and the potential in tile order.
+inline int ysplit_prepare(gpe_handle* h) {
+  if (!ysplit_eligible(h)) return GPE_OK;
+  if (!h->twy) {
+    std::vector<double2> t = twiddles(256);
+    HIPCHK(h, hipMalloc(&h->twy, 256 * sizeof(double2)));
+    HIPCHK(h, hipMemcpy(h->twy, t.data(), 256 * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  if (!h->kcolB) HIPCHK(h, hipMalloc(&h->kcolB, (size_t)h->nx * h->ny * sizeof(double2)));
+  if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
+  if (h->vp_gen != h->v_gen) {
+    HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
+                                           (unsigned)h->ysplit_na, h->stream));
+    h->vp_gen = h->v_gen;
+  }
+  return GPE_OK;
 }
 
 inline int read_state(gpe_handle* h, DevState* out) {
@@ -443,8 +490,72 @@ inline int run_block_slabs(gpe_handle* h, double time2) {
   return GPE_OK;
 }
 
+// The y-split 3D step applies: one component, no rotation, ny = 256, x length with a
+// y-split x sweep, z length with a 16-pencil-multiple tile, nz a multiple of 8.
+inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy; }
+
+// x-sweep parameters of the y-split step.
+inline SweepParams ysplit_params(gpe_handle* h) {
+  SweepParams p = base_params(h, 0, h->psi[0], h->Vp);
+  p.twy = h->twy;
+  p.gnx = (unsigned)h->nx;
+  p.gny = (unsigned)h->ny;
+  p.gnz = (unsigned)h->nz;
+  return p;
+}
+
+// One block of solve_split.m:53-78 as two sweeps per step (y factor over b with the x
+// sweep, y factor over a with the z sweep): same operations as run_block, the y
+// transform factored 16 x 16.
+inline int run_block_ysplit(gpe_handle* h, double time2) {
+  int rc;
+  if (h->vp_gen != h->v_gen) {
+    HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
+                                           (unsigned)h->ysplit_na, h->stream));
+    h->vp_gen = h->v_gen;
+  }
+  SweepParams px = ysplit_params(h);
+  SweepParams pz = base_params(h, 2, h->psi[0], h->V[0]);
+  pz.kcol = h->kcolB;
+  pz.twy = h->twy;
+  auto xs = [&](int seq, int* nb) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (h->prof) {
+      e0 = prof_event(h);
+      e1 = prof_event(h);
+      if (e0) hipEventRecord(e0, h->stream);
+    }
+    hipError_t e = gpe::launch_ysplit_x(h->nx, seq, h->ysplit_na, h->ysplit_pv, px, h->stream, nb);
+    if (h->prof && e0 && e1) {
+      hipEventRecord(e1, h->stream);
+      h->prof_recs.push_back({seq * 3 + 0, {e0, e1}});
+    }
+    if (e != hipSuccess) return fail(h, GPE_EHIP, "y-split x sweep (seq %d): %s", seq, hipGetErrorString(e));
+    return GPE_OK;
+  };
+  for (int i = 0; i < h->n_rec; ++i) {
+    if ((rc = xs(i > 0 ? gpe::S_YMID : gpe::S_YOPEN, nullptr))) return rc;
+    if ((rc = sweep(h, 2, gpe::S_YKIN, pz))) return rc;
+  }
+  FinParams f = fin_params(h, gpe::FIN_BLOCK);
+  int nb = 0;
+  if ((rc = xs(gpe::S_YCLOSEM, &nb))) return rc;
+  f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
+  for (int a = 1; a < 3; ++a) {
+    if ((rc = sweep(h, a, gpe::S_KE, base_params(h, a, h->psi[0], h->V[0]), &nb))) return rc;
+    f.cnt[gpe::SL_KX + a] = nb;
+  }
+  f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
+  f.NN0 = h->NN0;
+  f.tau = h->prm.decay_rate;
+  f.time2 = time2;
+  HIPCHK(h, gpe::launch_finalize(f, h->stream));
+  return GPE_OK;
+}
+
 inline int run_block(gpe_handle* h, double time2) {
   const bool rot = h->omega != 0.0;
+  if (ysplit_active(h)) return run_block_ysplit(h, time2);
   if (!rot && h->ndims == 3 && h->slabs > 1 && h->nz % h->slabs == 0) return run_block_slabs(h, time2);
   int rc;
   double2* psi = h->psi[0];

@@ -187,6 +187,65 @@ hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_
   }
 }
 
+#define GPE_DECL(N) hipError_t launch_ysplit_x_##N(int, int, int, const SweepParams&, hipStream_t, int*);
+GPE_DECL(64) GPE_DECL(128) GPE_DECL(256)
+#undef GPE_DECL
+
+bool ysplit_x_supported(int n) { return n == 64 || n == 128 || n == 256; }
+
+hipError_t launch_ysplit_x(int n, int seq, int na, int pv, const SweepParams& p, hipStream_t s, int* nblocks) {
+  switch (n) {
+    case 64: return launch_ysplit_x_64(seq, na, pv, p, s, nblocks);
+    case 128: return launch_ysplit_x_128(seq, na, pv, p, s, nblocks);
+    case 256: return launch_ysplit_x_256(seq, na, pv, p, s, nblocks);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Potential in the tile order of the y-split x sweeps (tiles of na values of a):
+// one thread per output element.
+__global__ void __launch_bounds__(256) ysplit_permute_v_kernel(const double* __restrict__ V, double* __restrict__ Vp,
+                                                               unsigned nx, unsigned na, unsigned total) {
+  const unsigned i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  const unsigned C = 16u * na, tpp = 16u / na;
+  const unsigned c = i % C, x = (i / C) % nx, tile = (i / C) / nx;   // tile = z*tpp + ap
+  const unsigned ap = tile % tpp, z = tile / tpp;
+  const unsigned y = na * ap + (c >> 4) + 16u * (c & 15u);
+  Vp[i] = V[y + 256u * x + 256u * nx * z];
+}
+
+hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na, hipStream_t s) {
+  const unsigned total = 256u * nx * nz;
+  hipLaunchKernelGGL(ysplit_permute_v_kernel, dim3((total + 255u) / 256u), dim3(256), 0, s, V, Vp, nx, na, total);
+  return hipGetLastError();
+}
+
+// One wavefront: value m of lane l is in[l + 64 m] (m < 2); out_f = forward row DFT,
+// out_i = inverse row DFT of the same input.
+__global__ void __launch_bounds__(64) rowdft_selftest_kernel(const double2* in, double2* out_f, double2* out_i,
+                                                             const double2* tw256) {
+  const int l = threadIdx.x;
+  RowFactors rf;
+  rf.init(l & 15, tw256, 16);
+  double2 f[2], g[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) f[m] = g[m] = in[l + 64 * m];
+  row_dft16_fwd<2>(f, rf);
+  row_dft16_inv<2>(g, rf);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    out_f[l + 64 * m] = f[m];
+    out_i[l + 64 * m] = g[m];
+  }
+}
+
+hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, const double2* tw256,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(rowdft_selftest_kernel, dim3(1), dim3(64), 0, s, in, out_f, out_i, tw256);
+  return hipGetLastError();
+}
+
 hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks, int var) {
   switch (n) {
 #define GPE_C(N) case N: return launch_sweep_##N(seq, colfast, var, p, s, nblocks);

@@ -36,7 +36,14 @@ enum Seq : int {
   S_SOPEN = 17,   // block-open half phase, + noise, forward
   S_SMID = 18,    // inverse, full phase, + noise, forward
   S_KINM = 19,    // S_KIN with the energy cut-off mask kk < ecut
-  S_NSEQ = 20
+  // 3D steps with a 256-point y axis split as 16 x 16 (y = a + 16 b): the b factor runs
+  // in the x sweeps (tiles of 2 a x 16 b x nx), the a factor in the z sweep, so a step
+  // is two sweeps; between them the state is x-spectral with the y factor over b applied
+  S_YKIN = 20,    // y factor over a (16 adjacent pencils), forward, x kcol[col]*kax[i], inverse, y factor back
+  S_YOPEN = 21,   // scale, block-open half phase, forward x, y factor over b
+  S_YMID = 22,    // y factor over b back, inverse x, full phase, forward x, y factor over b
+  S_YCLOSEM = 23, // y factor over b back, inverse x, full and closing half phase; block-end sums
+  S_NSEQ = 24
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -101,6 +108,9 @@ struct SweepParams {
   double noise_var;          // sqrt(T gamma ddt / weight)
   unsigned long long noise_step, noise_seed;   // Philox counter word 1-2 and key
   const int* kmax;           // S_KINM: largest |k| index of the mid axis kept in each column (-1: none)
+  // y-split 3D steps (S_Y*)
+  const double2* twy;        // y twiddles e^{-2 pi i j/256}
+  unsigned gnx, gny, gnz;    // grid sizes (x sweeps of S_YOPEN/S_YMID/S_YCLOSEM address the grid directly)
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).
@@ -143,5 +153,15 @@ hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipS
                         int var = 0);
 bool sweep_length_supported(int n);
 hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks);
+// x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM) for x length n; p.V is the
+// potential in tile order (launch_ysplit_permute_v)
+// na (1 or 2) values of a per tile; pv 0: the phase sweeps' plan, 1: Plan<n>.
+hipError_t launch_ysplit_x(int n, int seq, int na, int pv, const SweepParams& p, hipStream_t s, int* nblocks);
+bool ysplit_x_supported(int n);
+// Vp[(z*(16/na) + ap)*16 na*nx + x*16 na + c] = V[(na ap + c/16 + 16 (c%16)) + 256 x + 256 nx z]
+hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na,
+                                   hipStream_t s);
+hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, const double2* tw256,
+                                  hipStream_t s);
 
 }  // namespace gpe

@@ -12,6 +12,7 @@
 #include "phase_dev.h"
 #include "philox_dev.h"
 #include "reduce_dev.h"
+#include "rowdft_dev.h"
 #include "sweep.h"
 
 namespace gpe {
@@ -90,7 +91,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   const int c = COLFAST ? tid % C : tid / T;
   const int t = COLFAST ? tid / C : tid % T;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
-  if constexpr (SEQ == S_KIN || SEQ == S_KINM) {
+  if constexpr (SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN) {
     for (int i = tid; i < N; i += 256) twl[N + i] = p.kax[i];
     __syncthreads();  // single-pass plans have no exchange barrier before the first use
   }
@@ -158,6 +159,20 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+  } else if constexpr (SEQ == S_YKIN) {
+    // 16 adjacent pencils hold one y block (a = 0..15) in the 16 lanes of a row
+    // (C is a multiple of 16, lanes are consecutive pencils): the y factor over a runs
+    // across the row before the z transform and back after it
+    static_assert(COLFAST && C % 16 == 0, "y blocks must fill DPP rows");
+    RowFactorsT<true> rf;
+    rf.init(c & 15, twl, N / 16);   // W16^q = e^{-2 pi i q (N/16)/N}, from the staged twiddles
+    const double2 kc = p.kcol[cg];
+    row_dft16_fwd<E>(v, rf);
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
+    fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
+    row_dft16_inv<E>(v, rf);
   } else if constexpr (SEQ == S_KEX || SEQ == S_KE) {
     if constexpr (SEQ == S_KEX) {
       if (active) {
@@ -336,7 +351,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                  : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE || SEQ == S_CLOSEM) ? GPE_WAVES_PHASE : 1)))
     sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM ? 2 * N : N];
+  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN ? 2 * N : N];
   __shared__ double red[4];
   const unsigned b = C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const unsigned tile = p.reverse ? gridDim.x - 1u - b : b;

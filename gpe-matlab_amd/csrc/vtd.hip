@@ -71,6 +71,7 @@ int gpe_vtd_coeffs(gpe_handle* h, const double* c, int target) {
     HIPCHK(h, hipGetLastError());
     if (target) h->use_vmu = true;
     else h->have_V[comp] = true;
+    if (!target && comp == 0) ++h->v_gen;
   }
   return GPE_OK;
 }

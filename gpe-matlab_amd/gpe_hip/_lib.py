@@ -103,6 +103,7 @@ SIGNATURES = {
     "gpe_wisdom_get": [C.c_int, C.c_int, C.c_int],
     "gpe_wisdom_set": [C.c_int, C.c_int, C.c_int, C.c_int],
     "gpe_wisdom_forget": [],
+    "gpe_selftest_rowdft": [_DP, _DP, _DP],
 }
 OPTIONAL = set()
 
@@ -407,7 +408,7 @@ class Device:
 
 SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "kex", 7: "ke", 8: "itpa", 9: "itpb",
              10: "closem", 11: "rkin", 12: "ropen", 13: "rmid", 14: "rclosem", 15: "ritpa", 16: "ritpb",
-             17: "sopen", 18: "smid", 19: "kinm"}
+             17: "sopen", 18: "smid", 19: "kinm", 20: "ykin", 21: "yopen", 22: "ymid", 23: "yclosem"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
 
 
@@ -514,6 +515,19 @@ def wisdom_set(ndims, axis, n, variant):
 def wisdom_forget():
     """fftw('dwisdom', []): forget every recorded plan."""
     load().gpe_wisdom_forget()
+
+
+def selftest_rowdft(x):
+    """Forward and inverse 16-point DFT across DPP rows (test hook): x holds 128
+    complex values, entry l + 64 m is value m of lane l."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    assert x.size == 128
+    f = np.empty(128, dtype=np.complex128)
+    i = np.empty(128, dtype=np.complex128)
+    check(lib.gpe_selftest_rowdft(x.view(np.float64).ctypes.data_as(_DP), f.view(np.float64).ctypes.data_as(_DP),
+                                  i.view(np.float64).ctypes.data_as(_DP)), None, "gpe_selftest_rowdft")
+    return f, i
 
 
 def selftest_phase(er, ei):

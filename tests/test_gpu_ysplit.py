@@ -1,0 +1,201 @@
+"""The two-sweep 3D step of grids with a 256-point y axis (y = a + 16 b split).
+
+The y transform is factored 16 x 16: the factor over b runs inside the x sweep,
+the factor over a inside the z sweep, each as a radix-2 network across the 16 lanes
+of a DPP row.  Checked here: the row DFT against numpy's FFT (in its bit-reversed
+order), and GPEtask.solve_split on such grids against the FP64 oracle (rel-L2 <=
+1e-11) and against the four-sweep path of the same library (GPE_YSPLIT=0).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def two_sweep_step():
+    """Handles created inside these tests run the two-sweep step (it is opt-in)."""
+    old = os.environ.get("GPE_YSPLIT")
+    os.environ["GPE_YSPLIT"] = "1"
+    yield
+    if old is None:
+        del os.environ["GPE_YSPLIT"]
+    else:
+        os.environ["GPE_YSPLIT"] = old
+
+TOL = 1e-11
+BR4 = np.array([int(f"{r:04b}"[::-1], 2) for r in range(16)])
+
+
+def test_row_dft16_matches_numpy():
+    from gpe_hip import _lib
+    rng = np.random.Generator(np.random.PCG64(11))
+    x = rng.standard_normal(128) + 1j * rng.standard_normal(128)
+    f, i = _lib.selftest_rowdft(x)
+    for m in range(2):
+        for q in range(4):
+            lanes = np.arange(16) + 16 * q + 64 * m
+            X = np.fft.fft(x[lanes])
+            np.testing.assert_allclose(f[lanes], X[BR4], rtol=0, atol=1e-13 * np.abs(X).max())
+            # inverse: lane r holds X_{bitrev4(r)} on entry
+            Y = np.empty(16, dtype=complex)
+            Y[BR4] = x[lanes]
+            y = np.fft.ifft(Y) * 16
+            np.testing.assert_allclose(i[lanes], y, rtol=0, atol=1e-13 * np.abs(y).max())
+    # a unit impulse in lane 3 of row 1: forward gives W16^{3 k} in bit-reversed order
+    e = np.zeros(128, dtype=complex)
+    e[16 + 3] = 1.0
+    f, _ = _lib.selftest_rowdft(e)
+    k = BR4
+    np.testing.assert_allclose(f[16:32], np.exp(-2j * np.pi * ((3 * k) % 16) / 16), rtol=0, atol=1e-15)
+    assert np.all(f[:16] == 0) and np.all(f[32:] == 0)
+
+
+def _pair(n, xmax):
+    import gpe_hip
+    return (gpe_hip.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2]),
+            oracle.grid3d(xmax[0], n[0], xmax[1], n[1], xmax[2], n[2]))
+
+
+def V3(X, Y, Z):
+    return 0.5 * (X ** 2 + 1.5 * Y ** 2 + 3.0 * Z ** 2)
+
+
+def _make(G, mod, phi, **props):
+    t = mod.GPEtask(G, V3)
+    t.g, t.Ntotal = 0.6, 10.0
+    for k, v in props.items():
+        setattr(t, k, v)
+    t.init_state = phi.copy()
+    return t
+
+
+def _phi(G, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r2 = G.mesh.x ** 2 + G.mesh.y ** 2 + G.mesh.z ** 2
+    phi = np.exp(-r2 / 3) * np.exp(1j * rng.uniform(0, 0.2, G.size))
+    return phi * np.exp(1j * np.arctan2(G.mesh.x - 0.4, G.mesh.y + 0.2))
+
+
+def _four_sweep(fn):
+    os.environ["GPE_YSPLIT"] = "0"
+    try:
+        return fn()
+    finally:
+        os.environ["GPE_YSPLIT"] = "1"
+
+
+CASES = [((64, 256, 64), (6, 9, 5)), ((128, 256, 64), (7, 10, 5)), ((64, 256, 128), (6, 8, 7)),
+         ((256, 256, 64), (9, 9, 5))]
+
+
+@pytest.mark.parametrize("na,plan", [(1, 1), (1, 0), (2, 1), (2, 0)])
+def test_ysplit_tile_and_plan_options_match_oracle(na, plan):
+    import gpe_hip
+    n, xmax = (128, 256, 64), (7, 10, 5)
+    G, O = _pair(n, xmax)
+    phi = _phi(G, 21)
+    os.environ["GPE_YSPLIT_NA"], os.environ["GPE_YSPLIT_PLAN"] = str(na), str(plan)
+    try:
+        tg = _make(G, gpe_hip, phi, gamma=0.01)
+        tg._device()
+    finally:
+        del os.environ["GPE_YSPLIT_NA"], os.environ["GPE_YSPLIT_PLAN"]
+    to = _make(O, oracle, phi, gamma=0.01)
+    err = rel_l2(tg.solve_split(0.002, 10, 2), to.solve_split(0.002, 10, 2))
+    print(f"y-split na={na} plan={plan}: rel-L2 vs oracle {err:.3e}", flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("n,xmax", CASES)
+@pytest.mark.parametrize("gamma", [0.0, 0.02])
+def test_ysplit_solve_split_matches_oracle(n, xmax, gamma):
+    import gpe_hip
+    G, O = _pair(n, xmax)
+    phi = _phi(G, sum(n))
+    tg = _make(G, gpe_hip, phi, gamma=gamma)
+    to = _make(O, oracle, phi, gamma=gamma)
+    pg = tg.solve_split(0.002, 10, 2)
+    po = to.solve_split(0.002, 10, 2)
+    err = rel_l2(pg, po)
+    print(f"y-split {n} gamma={gamma}: rel-L2 vs oracle {err:.3e}", flush=True)
+    assert err < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=TOL, atol=0)
+    # the step ran as two sweeps: one more outer iteration under the profiler
+    dev = tg._device()
+    dev.profile_enable(True)
+    tg.solve_split(0.002, 10, 3)
+    prof = dev.profile_read()
+    dev.profile_enable(False)
+    seqs = {s for (s, _ax) in prof}
+    assert {20, 21, 22, 23} <= seqs and not ({4, 0, 1, 2} & seqs), seqs
+
+
+def test_ysplit_equals_four_sweep_path_and_resumes():
+    import gpe_hip
+    n, xmax = (64, 256, 64), (6, 9, 5)
+    G, O = _pair(n, xmax)
+    phi = _phi(G, 5)
+    props = dict(gamma=0.01, decay_rate=0.05, mu_init=0.0)
+    ty = _make(G, gpe_hip, phi, **props)
+    ty.solve_split(0.002, 20, 2)
+    py = ty.solve_split(0.002, 20, 4)           # resume from current_iter = 2
+    tf = _four_sweep(lambda: _make(G, gpe_hip, phi, **props))
+    _four_sweep(lambda: tf.solve_split(0.002, 20, 2))
+    pf = _four_sweep(lambda: tf.solve_split(0.002, 20, 4))
+    to = _make(O, oracle, phi, **props)
+    to.solve_split(0.002, 20, 2)                # a resumed run recomputes mu from the state
+    po = to.solve_split(0.002, 20, 4)           # (solve_split.m:41), as the reference does
+    e_yf, e_yo, e_fo = rel_l2(py, pf), rel_l2(py, po), rel_l2(pf, po)
+    print(f"y-split vs 4-sweep {e_yf:.3e}, vs oracle {e_yo:.3e}; 4-sweep vs oracle {e_fo:.3e}", flush=True)
+    assert e_yo < TOL and e_fo < TOL and e_yf < TOL
+    assert np.allclose(ty.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(ty.history["n"], to.history["n"], rtol=TOL, atol=0)
+
+
+def test_ysplit_graph_replay_matches_direct_launches():
+    import gpe_hip
+    n, xmax = (128, 256, 64), (7, 9, 5)
+    G, _ = _pair(n, xmax)
+    phi = _phi(G, 8)
+    ta = _make(G, gpe_hip, phi, gamma=0.02)
+    pa = ta.solve_split(0.002, 10, 4)
+    old = os.environ.get("GPE_NO_GRAPH")
+    os.environ["GPE_NO_GRAPH"] = "1"
+    try:
+        tb = _make(G, gpe_hip, phi, gamma=0.02)
+        pb = tb.solve_split(0.002, 10, 4)
+    finally:
+        if old is None:
+            del os.environ["GPE_NO_GRAPH"]
+        else:
+            os.environ["GPE_NO_GRAPH"] = old
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(ta.history["mu"], tb.history["mu"])
+
+
+def test_ysplit_time_dependent_potential():
+    """A host Vtd re-uploads V every block; the tile-order copy must follow it."""
+    import gpe_hip
+    n, xmax = (64, 256, 64), (6, 9, 5)
+    G, O = _pair(n, xmax)
+    phi = _phi(G, 9)
+
+    def vtd(task, t):
+        return 0.3 * np.sin(7.0 * t) * task.grid.mesh.x + 0.1 * np.cos(5.0 * t) * task.grid.mesh.y ** 2
+
+    tg = _make(G, gpe_hip, phi, gamma=0.0)
+    to = _make(O, oracle, phi, gamma=0.0)
+    tg.Vtd = vtd
+    to.Vtd = vtd
+    pg = tg.solve_split(0.002, 10, 3)
+    po = to.solve_split(0.002, 10, 3)
+    err = rel_l2(pg, po)
+    print(f"y-split with host Vtd: rel-L2 vs oracle {err:.3e}", flush=True)
+    assert err < TOL
