@@ -10,6 +10,7 @@
 // is captured once into a hipGraph and replayed.  Per-trajectory scalars (mu,
 // renormalisation, ncur) live in a DevState array and are reduced on the device.
 #include <algorithm>
+#include <cmath>
 #include <map>
 
 #include "batch.h"
@@ -363,7 +364,27 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
   std::map<std::pair<int, int>, std::vector<int>> by_shape;
   for (int i = 0; i < ntraj; ++i) by_shape[{b->tr[i].nx, b->tr[i].ny}].push_back(i);
   std::vector<std::pair<double, std::vector<int>>> bins;
-  for (auto& kv : by_shape) bins.push_back({(double)kv.first.first * kv.first.second * kv.second.size(), kv.second});
+  double total_pts = 0;
+  for (auto& kv : by_shape) {
+    bins.push_back({(double)kv.first.first * kv.first.second * kv.second.size(), kv.second});
+    total_pts += bins.back().first;
+  }
+  // a shape holding well over a quarter of the points is cut into chains of about a
+  // quarter each, so no single chain is the critical path (GPE_BATCH_SPLIT=0: off)
+  const char* bsp = getenv("GPE_BATCH_SPLIT");
+  if (!(bsp && atoi(bsp) == 0)) {
+    std::vector<std::pair<double, std::vector<int>>> cut;
+    for (auto& bn : bins) {
+      const int cnt = (int)bn.second.size();
+      const int k = std::max(1, std::min(cnt, (int)std::floor(bn.first / (total_pts / 4.0) + 0.5)));
+      const double per = bn.first / cnt;
+      for (int q = 0; q < k; ++q) {
+        const int lo = (int)((long long)cnt * q / k), hi = (int)((long long)cnt * (q + 1) / k);
+        cut.push_back({per * (hi - lo), std::vector<int>(bn.second.begin() + lo, bn.second.begin() + hi)});
+      }
+    }
+    bins.swap(cut);
+  }
   std::sort(bins.begin(), bins.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
   while (bins.size() > 4) {
     auto last = bins.back();
