@@ -18,6 +18,6 @@ struct AdjParams {
   double h;              // real time step ddt (gamma = 0: dt = i ddt)
 };
 
-hipError_t launch_adj_x(int n, int seq, const AdjParams& p, hipStream_t s, int* nblocks);
+hipError_t launch_adj_x(int n, int seq, int var, const AdjParams& p, hipStream_t s, int* nblocks);
 
 }  // namespace gpe

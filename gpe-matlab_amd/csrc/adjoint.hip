@@ -17,14 +17,14 @@ using namespace gpe_detail;
 
 namespace gpe {
 
-#define GPE_DECL(N) hipError_t launch_adj_x_##N(int, const AdjParams&, hipStream_t, int*);
+#define GPE_DECL(N) hipError_t launch_adj_x_##N(int, int, const AdjParams&, hipStream_t, int*);
 GPE_DECL(8) GPE_DECL(12) GPE_DECL(16) GPE_DECL(24) GPE_DECL(32) GPE_DECL(48) GPE_DECL(64) GPE_DECL(96)
 GPE_DECL(128) GPE_DECL(192) GPE_DECL(256) GPE_DECL(384) GPE_DECL(512)
 #undef GPE_DECL
 
-hipError_t launch_adj_x(int n, int seq, const AdjParams& p, hipStream_t s, int* nblocks) {
+hipError_t launch_adj_x(int n, int seq, int var, const AdjParams& p, hipStream_t s, int* nblocks) {
   switch (n) {
-#define GPE_C(N) case N: return launch_adj_x_##N(seq, p, s, nblocks);
+#define GPE_C(N) case N: return launch_adj_x_##N(seq, var, p, s, nblocks);
     GPE_C(8) GPE_C(12) GPE_C(16) GPE_C(24) GPE_C(32) GPE_C(48) GPE_C(64) GPE_C(96) GPE_C(128) GPE_C(192)
     GPE_C(256) GPE_C(384) GPE_C(512)
 #undef GPE_C
@@ -288,7 +288,9 @@ int adj_x(gpe_handle* h, int seq) {
     e1 = prof_event(h);
     if (e0) hipEventRecord(e0, h->stream);
   }
-  hipError_t e = gpe::launch_adj_x(a.n, seq, p, h->stream, nullptr);
+  const char* av = getenv("GPE_ADJ_VAR");   // -1 (unset): follow the x-axis plan of the forward sweeps
+  const int var = av ? atoi(av) : h->var[0];
+  hipError_t e = gpe::launch_adj_x(a.n, seq, var, p, h->stream, nullptr);
   if (h->prof && e0 && e1) {
     hipEventRecord(e1, h->stream);
     h->prof_recs.push_back({(seq == gpe::A_MID ? gpe::S_MID : seq == gpe::A_OPEN ? gpe::S_OPEN : gpe::S_CLOSE) * 3,

@@ -11,6 +11,8 @@
 // applies the forward x-FFT; A_MID does inverse x-FFT, undoes one full phase,
 // forward x-FFT; A_OPEN does inverse x-FFT and undoes the block-open phase
 // (solve_split.m:55-56).  dL/dV is accumulated in gV (read-modify-write).
+#include <type_traits>
+
 #include "adjoint.h"
 #include "fft_dev.h"
 #include "phase_dev.h"
@@ -35,9 +37,15 @@ __device__ __forceinline__ void undo_phase(double2& w, double2& lam, double& gv,
   w = make_double2(c * w.x - s * w.y, c * w.y + s * w.x);
 }
 
-template <int N, int C, int SEQ>
+// Plan of the adjoint x sweep: Plan<N>, or with var = 1 the length's second plan
+// (AltPlan, fewer elements per thread where it has one), chosen by the handle's x-axis
+// plan variant.
+template <int N, int VAR>
+using AdjPlan = typename std::conditional<VAR == 1 && AltPlan<N>::has, typename AltPlan<N>::type, Plan<N>>::type;
+
+template <int N, int C, int SEQ, int VAR>
 __global__ void __launch_bounds__(256) adj_x_kernel(const AdjParams p) {
-  using PL = Plan<N>;
+  using PL = AdjPlan<N, VAR>;
   constexpr int T = PL::T, E = PL::E;
   constexpr bool COLFAST = true;
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
@@ -97,19 +105,27 @@ __global__ void __launch_bounds__(256) adj_x_kernel(const AdjParams p) {
 #define GPE_CAT2(a, b) a##b
 #define GPE_CAT(a, b) GPE_CAT2(a, b)
 
-hipError_t GPE_CAT(launch_adj_x_, GPE_N)(int seq, const AdjParams& p, hipStream_t s, int* nblocks) {
-  constexpr int N = GPE_N;
-  constexpr int C = 256 / Plan<N>::T;
+template <int N, int VAR>
+static hipError_t launch_adj_var(int seq, const AdjParams& p, hipStream_t s, int* nblocks) {
+  constexpr int C = 256 / AdjPlan<N, VAR>::T;
   const long long nb = ((long long)p.ncols + C - 1) / C;
   if (nblocks) *nblocks = (int)nb;
   if (nb <= 0) return hipSuccess;
   switch (seq) {
-    case A_OPEN: hipLaunchKernelGGL((adj_x_kernel<N, C, A_OPEN>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
-    case A_MID: hipLaunchKernelGGL((adj_x_kernel<N, C, A_MID>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
-    case A_CLOSE: hipLaunchKernelGGL((adj_x_kernel<N, C, A_CLOSE>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
+    case A_OPEN: hipLaunchKernelGGL((adj_x_kernel<N, C, A_OPEN, VAR>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
+    case A_MID: hipLaunchKernelGGL((adj_x_kernel<N, C, A_MID, VAR>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
+    case A_CLOSE: hipLaunchKernelGGL((adj_x_kernel<N, C, A_CLOSE, VAR>), dim3((unsigned)nb), dim3(256), 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t GPE_CAT(launch_adj_x_, GPE_N)(int seq, int var, const AdjParams& p, hipStream_t s, int* nblocks) {
+  constexpr int N = GPE_N;
+  if constexpr (AltPlan<N>::has) {
+    if (var == 1) return launch_adj_var<N, 1>(seq, p, s, nblocks);
+  }
+  return launch_adj_var<N, 0>(seq, p, s, nblocks);
 }
 
 }  // namespace gpe

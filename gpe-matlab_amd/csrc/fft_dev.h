@@ -218,6 +218,13 @@ GPE_PLAN_NAMED(Plan384x24, 384, 24, 3, 12, 8, 4)
 GPE_PLAN_NAMED(Plan512x8, 512, 8, 3, 8, 8, 8)
 #undef GPE_PLAN_NAMED
 
+// Second plan of a length (plan variant 1), where one exists (256 has its own rule).
+template <int N> struct AltPlan { static constexpr bool has = false; using type = Plan<N>; };
+template <> struct AltPlan<64> { static constexpr bool has = true; using type = Plan64x8; };
+template <> struct AltPlan<128> { static constexpr bool has = true; using type = Plan128x8; };
+template <> struct AltPlan<384> { static constexpr bool has = true; using type = Plan384x24; };
+template <> struct AltPlan<512> { static constexpr bool has = true; using type = Plan512x8; };
+
 // One Stockham pass on the canonical layout.  Butterfly j = t + T*b takes
 // elements j + r*N/R (register b + r*B), twiddles them by W_{NS R}^{r (j mod NS)}
 // and applies a radix-R DFT; output r of butterfly j belongs at

@@ -31,12 +31,6 @@ constexpr bool tunable_seq(int s) {
          s == S_CLOSE || s == S_CLOSEM;
 }
 
-// Second plan of a length (variant 1), where one exists.
-template <int N> struct AltPlan { static constexpr bool has = false; using type = Plan<N>; };
-template <> struct AltPlan<64> { static constexpr bool has = true; using type = Plan64x8; };
-template <> struct AltPlan<128> { static constexpr bool has = true; using type = Plan128x8; };
-template <> struct AltPlan<384> { static constexpr bool has = true; using type = Plan384x24; };
-template <> struct AltPlan<512> { static constexpr bool has = true; using type = Plan512x8; };
 template <int N> constexpr bool has_variant() { return N == 256 || AltPlan<N>::has; }
 
 // FFT plan of a sweep.  Variant 0: Plan<N>, except 8.8.4 for the phase sweeps of
