@@ -267,11 +267,21 @@ __device__ __forceinline__ int pass_out_index(int t, int b, int r) {
 // LDS exchange helpers.  COLFAST: element i of column c lives at i*C + c (lanes
 // are consecutive columns, conflict-free).  TFAST: column c owns a padded row
 // of N + N/16 slots (lanes are consecutive t within one column).
+// COLFAST with C = 8 or 4: the 32 doubles of one 64-bank line hold RL = 32/C rows, and
+// a row's slot in its line is XOR-ed with (i / 8) mod RL.  Without it the first
+// Stockham pass of an 8-element plan writes rows 8t + r from the lane groups t of a
+// wave, which all fall in the same slot (8- or 16-way bank conflicts); with it they
+// spread over the line (2-way, the minimum for 512 bytes per instruction).
 template <int N, int C, bool COLFAST>
 struct Lds {
   static constexpr int PITCH = N + N / 16 + 1;
   static constexpr int SIZE = COLFAST ? N * C : C * PITCH;
+  static constexpr int RL = 32 / C;
   __device__ __forceinline__ static int at(int c, int i) {
+    if (COLFAST && (C == 8 || C == 4)) {
+      const unsigned u = (unsigned)i;   // rows are non-negative: shifts and masks, no signed division
+      return (int)((u / RL) * 32u + (((u % RL) ^ ((u >> 3) % RL)) * (unsigned)C)) + c;
+    }
     return COLFAST ? i * C + c : c * PITCH + i + (i >> 4);
   }
 };
