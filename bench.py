@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
+    ap.add_argument("--no-tune", action="store_true", help="c4: keep the default FFT plans (no plan tuner)")
     ap.add_argument("--ysplit", action="store_true",
                     help="c2: run the two-sweep step (256-point y transform split 16 x 16, GPE_YSPLIT=1)")
     ap.add_argument("--omega", type=float, default=0.0,
@@ -386,6 +387,11 @@ def bench_c4(args):
     npts = sum(int(np.prod(gr.size)) for gr in grids)
     ddt, ni = 0.004, 100
     b = _lib.Batch(grids)
+    # FFT plans of the ensemble's (axis, length) pairs, measured on its own sweeps; the
+    # per-trajectory handles below read the same plan wisdom
+    tune_us = {} if args.no_tune else b.tune()
+    log("c4 plan tuner (us per ensemble step, variant 0 / 1): "
+        + ", ".join(f"{'xy'[a]}{n}: {t0:.0f}/{t1:.0f}" for (a, n), (t0, t1) in sorted(tune_us.items())))
     for i, (gr, V, psi, pr) in enumerate(ens):
         b.set(i, _lib.make_params(1, pr["g"], 0.0, 0.0, pr["Ntotal"], 0.0, 10), V, psi)
     b.split_begin(ddt, ni, True)
@@ -427,6 +433,7 @@ def bench_c4(args):
                                  "V=0.5(X^2+lam^2 Y^2) lam~U[1,2], TF + vortex |r0|<5, dt=0.004, graph per 100 steps",
                      "points": npts},
           per_trajectory_launch_rate=round(single_rate, 1), batched_vs_per_trajectory=round(traj_rate / single_rate, 2),
+          plan_tuner_us={f"{'xy'[a]}{n}": [round(t0, 1), round(t1, 1)] for (a, n), (t0, t1) in sorted(tune_us.items())},
           ensemble_alg_GBps=round(gbps, 1), batched_equals_single_bitwise=bool(same))
 
 
