@@ -202,8 +202,8 @@ int gpe_detect_core_array(gpe_handle* h, const double* phi, int ny, int nx, cons
                           int refine, int cap, double* resp, int* np, double* resm, int* nm);
 
 /* ------------------------------------------------------------------ adjoint
- * (no reference counterpart: north-star form 3, the gradient entry point that
- * plays the torch.autograd.Function.backward role).
+ * (no reference counterpart: the differentiation form of the split step, a gradient
+ * entry point that plays the torch.autograd.Function.backward role).
  * Control potential V(x, t) = V0(x) + sum_k c[k + nbasis*m] G_k(x), where m is the
  * interval of the block holding t (V is evaluated at block starts,
  * solve_split.m:54; intervals are steps_per_interval steps long).  The forward
@@ -226,7 +226,7 @@ int gpe_adjoint_grad(gpe_handle* h, const gpe_control_args* a, const double* coe
                      double* loss, double* grad, double* recon_err);
 
 /* ------------------------------------------------------------------ Rabi adjoint
- * (no reference counterpart: north-star form 3 on the two-component step).
+ * (no reference counterpart: the differentiation form on the two-component step).
  * Forward model: GPEtaskSpinor.solve_split (lib/@GPEtaskSpinor/solve_split.m) with
  * gamma = 0, nlincpl = 0, Ntotal > 0, nsteps steps of ddt in blocks of n_recalc,
  * and the coupling replaced by Omega(x) = omega0 exp(i k x).  Loss: the final
@@ -252,7 +252,7 @@ int gpe_rabi_forward(gpe_handle* h, const gpe_rabi_args* a, double* loss, double
 int gpe_rabi_grad(gpe_handle* h, const gpe_rabi_args* a, double* loss, double* grad);
 
 /* ------------------------------------------------------------------ ensemble
- * (no reference counterpart: north-star form 2).  A batch advances many
+ * (no reference counterpart: the batched form).  A batch advances many
  * independent 2D GPEtask.solve_split runs together; each trajectory has its own
  * grid (lengths may differ), potential, state and gpe_params (g, gamma, Ntotal > 0,
  * mu_init, decay_rate; one common n_recalc).  Every sweep is one launch per

@@ -2,7 +2,7 @@
 
 Every test runs the HIP path through the C ABI (gpe_hip -> libgpehip.so) and the
 oracle restatement on the same seeded inputs.  Tolerance: relative L2 <= 1e-11 on
-the final state, scalars to 1e-11 relative unless stated.
+the final state (the required bar), scalars to 1e-11 relative unless stated.
 """
 import numpy as np
 import pytest
