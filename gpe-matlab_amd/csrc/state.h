@@ -1,5 +1,8 @@
 // Block-boundary kernels: second-stage reductions and device-side scalars.
 #pragma once
+#include <array>
+#include <vector>
+
 #include "sweep.h"
 
 namespace gpe {
@@ -51,6 +54,8 @@ int wisdom_get(int ndims, int axis, int n);
 void wisdom_put(int ndims, int axis, int n, int var);
 bool wisdom_put_if_absent(int ndims, int axis, int n, int var);
 void wisdom_clear();
+// every recorded (ndims, axis, n, variant)
+std::vector<std::array<int, 4>> wisdom_entries();
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
 
 }  // namespace gpe

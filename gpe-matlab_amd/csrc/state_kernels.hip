@@ -1,9 +1,11 @@
 // Block-boundary kernels: second-stage reductions, device-side scalar updates
 // (norm, renormalisation, chemical potential) and small elementwise passes.
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "fft_dev.h"
 #include "phase_dev.h"
@@ -169,6 +171,14 @@ bool wisdom_put_if_absent(int ndims, int axis, int n, int var) {
 void wisdom_clear() {
   std::lock_guard<std::mutex> lk(g_wisdom_mu);
   g_wisdom.clear();
+}
+
+std::vector<std::array<int, 4>> wisdom_entries() {
+  std::lock_guard<std::mutex> lk(g_wisdom_mu);
+  std::vector<std::array<int, 4>> out;
+  for (auto& kv : g_wisdom)
+    out.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), kv.second});
+  return out;
 }
 
 #define GPE_DECL(N) hipError_t launch_spinor_sweep_##N(int, const SpinorParams&, hipStream_t, int*);

@@ -13,7 +13,10 @@
 // created later read that table, so all runs of one process on one grid shape use the
 // same plans and round identically.
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include "handle.h"
@@ -156,6 +159,34 @@ int gpe_wisdom_set(int ndims, int axis, int n, int variant) {
 
 int gpe_wisdom_forget(void) {
   gpe::wisdom_clear();
+  return GPE_OK;
+}
+
+// Text form "ndims axis n variant" per line (FFTW's wisdom export/import).
+int gpe_wisdom_export(char* buf, int len, int* needed) {
+  std::string s;
+  for (const auto& e : gpe::wisdom_entries())
+    s += std::to_string(e[0]) + " " + std::to_string(e[1]) + " " + std::to_string(e[2]) + " " + std::to_string(e[3]) + "\n";
+  if (needed) *needed = (int)s.size() + 1;
+  if (!buf || len < (int)s.size() + 1) return buf ? GPE_EINVAL : GPE_OK;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return GPE_OK;
+}
+
+int gpe_wisdom_import(const char* text) {
+  if (!text) return GPE_EINVAL;
+  std::vector<std::array<int, 4>> rows;
+  const char* p = text;
+  while (*p) {
+    int nd, ax, n, v, used = 0;
+    while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') ++p;
+    if (!*p) break;
+    if (std::sscanf(p, "%d %d %d %d%n", &nd, &ax, &n, &v, &used) != 4 || used <= 0) return GPE_EINVAL;
+    if ((nd != 2 && nd != 3) || ax < 0 || ax >= nd || v < 0 || v >= gpe::plan_variants(n)) return GPE_EINVAL;
+    rows.push_back({nd, ax, n, v});
+    p += used;
+  }
+  for (const auto& r : rows) gpe::wisdom_put(r[0], r[1], r[2], r[3]);   // all or nothing
   return GPE_OK;
 }
 

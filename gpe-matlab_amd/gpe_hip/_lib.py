@@ -104,6 +104,8 @@ SIGNATURES = {
     "gpe_wisdom_set": [C.c_int, C.c_int, C.c_int, C.c_int],
     "gpe_wisdom_forget": [],
     "gpe_selftest_rowdft": [_DP, _DP, _DP],
+    "gpe_wisdom_export": [C.c_char_p, C.c_int, _IP],
+    "gpe_wisdom_import": [C.c_char_p],
 }
 OPTIONAL = set()
 
@@ -515,6 +517,25 @@ def wisdom_set(ndims, axis, n, variant):
 def wisdom_forget():
     """fftw('dwisdom', []): forget every recorded plan."""
     load().gpe_wisdom_forget()
+
+
+def wisdom_export():
+    """Recorded plans as text, one "ndims axis n variant" line each (fftw('wisdom'))."""
+    lib = load()
+    need = C.c_int()
+    lib.gpe_wisdom_export(None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value + 64)
+    rc = lib.gpe_wisdom_export(buf, len(buf), C.byref(need))
+    if rc != GPE_OK:
+        raise GPELibraryError(f"gpe_wisdom_export: {ERRNAMES.get(rc, rc)}")
+    return buf.value.decode()
+
+
+def wisdom_import(text):
+    """Record the plans of an exported text (all lines, or none if one is malformed)."""
+    rc = load().gpe_wisdom_import(text.encode())
+    if rc != GPE_OK:
+        raise GPELibraryError(f"gpe_wisdom_import: {ERRNAMES.get(rc, rc)}")
 
 
 def selftest_rowdft(x):

@@ -298,6 +298,12 @@ int gpe_wisdom_get(int ndims, int axis, int n);
 int gpe_wisdom_set(int ndims, int axis, int n, int variant);
 /* fftw('dwisdom', []): forget every recorded plan (handles keep the plans they hold). */
 int gpe_wisdom_forget(void);
+/* fftw('wisdom') export / import: one line "ndims axis n variant" per recorded plan.
+ * gpe_wisdom_export writes the NUL-terminated text into buf (len bytes) and the size it
+ * needs into *needed (buf may be NULL to query it); gpe_wisdom_import records every
+ * line of text, or none if any line is malformed. */
+int gpe_wisdom_export(char* buf, int len, int* needed);
+int gpe_wisdom_import(const char* text);
 
 /* Measurement hooks (no reference counterpart): time every sweep launch with
  * HIP events recorded on the handle's own stream.  gpe_profile_read fills

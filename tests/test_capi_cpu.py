@@ -81,3 +81,29 @@ def test_profile_names_cover_every_sweep_sequence():
     assert sorted(_lib.SEQ_NAMES) == list(range(nseq))
     import bench
     assert set(_lib.SEQ_NAMES.values()) <= set(bench.SEQ_BYTES)
+
+
+def test_plan_wisdom_set_export_import_roundtrip(lib):
+    """The plan wisdom is host-side state: set, export, forget and import need no GPU."""
+    lib.gpe_wisdom_get.restype = C.c_int
+    lib.gpe_wisdom_export.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    lib.gpe_wisdom_import.argtypes = [C.c_char_p]
+    assert lib.gpe_wisdom_forget() == 0
+    assert lib.gpe_wisdom_get(3, 0, 256) == -1
+    assert lib.gpe_wisdom_set(3, 0, 256, 1) == 0
+    assert lib.gpe_wisdom_set(2, 1, 384, 1) == 0
+    assert lib.gpe_wisdom_set(2, 1, 192, 1) != 0          # 192 has one plan
+    assert lib.gpe_wisdom_set(2, 2, 128, 0) != 0          # axis out of range for 2D
+    need = C.c_int()
+    assert lib.gpe_wisdom_export(None, 0, C.byref(need)) == 0
+    buf = C.create_string_buffer(need.value)
+    assert lib.gpe_wisdom_export(buf, len(buf), C.byref(need)) == 0
+    text = buf.value.decode()
+    assert sorted(text.split("\n")[:-1]) == ["2 1 384 1", "3 0 256 1"]
+    assert lib.gpe_wisdom_forget() == 0
+    assert lib.gpe_wisdom_get(3, 0, 256) == -1
+    assert lib.gpe_wisdom_import(b"3 0 256 1\n2 1 384 x\n") != 0   # malformed: nothing recorded
+    assert lib.gpe_wisdom_get(3, 0, 256) == -1
+    assert lib.gpe_wisdom_import(text.encode()) == 0
+    assert lib.gpe_wisdom_get(3, 0, 256) == 1 and lib.gpe_wisdom_get(2, 1, 384) == 1
+    assert lib.gpe_wisdom_forget() == 0
