@@ -1,0 +1,88 @@
+"""Checks at the full sizes of the benchmark workloads (256^3 scalar, 192^3 two-component).
+
+At these sizes the oracle is only affordable for a few steps, so besides a 10-step
+parity run the tests use properties that hold at any size:
+- with gamma = 0 the split step conserves the norm up to rounding (solve_split.m:55-64
+  applies unit-modulus phases and a unitary kinetic step; no renormalisation);
+- with a fixed mu (mu_init > 0) a block run with -ddt undoes a block run with +ddt
+  (the open half phase uses the density the close half phase left);
+- the Rabi step of the two-component solver (GPEtaskSpinor/solve_split.m:61-65) is
+  unitary for a real time step, so the total norm is conserved.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _c2_task(mod, **props):
+    from gpe_hip import workloads
+    grid, V, psi0, p = workloads.c2_state(n=256)
+    G = grid if mod.__name__.startswith("gpe_hip") else oracle.grid3d(12, 256, 12, 256, 12, 256)
+    t = mod.GPEtask(G, V)
+    t.Ntotal, t.g, t.gamma = p["Ntotal"], p["g"], p["gamma"]
+    for k, v in props.items():
+        setattr(t, k, v)
+    t.init_state = psi0
+    return t, psi0, grid
+
+
+def test_c2_full_size_parity_10_steps():
+    import gpe_hip
+    tg, _, _ = _c2_task(gpe_hip)
+    to, _, _ = _c2_task(oracle)
+    pg = tg.solve_split(0.002, 10, 1)
+    po = to.solve_split(0.002, 10, 1)
+    err = rel_l2(pg, po)
+    print(f"256^3 gamma=0.01, 10 steps: rel-L2 vs oracle {err:.3e}", flush=True)
+    assert err < 1e-11
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=1e-11, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
+
+
+def test_c2_full_size_norm_conserved_without_damping():
+    import gpe_hip
+    tg, psi0, grid = _c2_task(gpe_hip, gamma=0.0)
+    n0 = grid.integrate(np.abs(psi0) ** 2)
+    p = tg.solve_split(0.002, 50, 2)
+    n1 = grid.integrate(np.abs(p) ** 2)
+    drift = abs(n1 - n0) / n0
+    print(f"256^3 gamma=0, 100 steps: relative norm drift {drift:.2e}", flush=True)
+    assert drift < 1e-12
+    assert np.allclose(tg.history["n"], n0, rtol=1e-12, atol=0)
+
+
+def test_c2_full_size_time_reversal():
+    import gpe_hip
+    tg, psi0, _ = _c2_task(gpe_hip, gamma=0.0, mu_init=95.0)
+    tg.solve_split(0.002, 10, 2)
+    back = tg.solve_split(-0.002, 10, 4)          # resumes at current_iter = 2, runs 2 blocks back
+    err = rel_l2(back, psi0)
+    print(f"256^3: 20 steps forward, 20 back, rel-L2 to the initial state {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
+def test_c5_full_size_spinor_norm_conserved():
+    import gpe_hip
+    from gpe_hip import workloads
+    n = 192
+    G = gpe_hip.grid3d(12, n, 12, n, 12, n)
+    V = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2 + G.mesh.z ** 2)
+    tf, _ = workloads.thomas_fermi(G, V, 1e6, 0.02)
+    rng = np.random.Generator(np.random.PCG64(5))
+    a = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    b = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
+    t = gpe_hip.GPEtaskSpinor(G, [V, V])
+    t.g, t.coupling, t.Ntotal = np.array([[0.02, 0.018], [0.018, 0.02]]), 0.5 * np.exp(2j * G.mesh.x), 1e6
+    t.init_state = [a, b]
+    n0 = G.integrate(np.abs(a) ** 2) + G.integrate(np.abs(b) ** 2)
+    p = t.solve_split(0.002, 10, 2)
+    n1 = G.integrate(np.abs(p[0]) ** 2) + G.integrate(np.abs(p[1]) ** 2)
+    f1 = G.integrate(np.abs(p[0]) ** 2) / n1
+    drift = abs(n1 - n0) / n0
+    print(f"192^3 spinor, 20 steps: relative norm drift {drift:.2e}, component-1 fraction {f1:.6f}", flush=True)
+    assert drift < 1e-12
+    assert abs(f1 - 0.5) > 1e-7          # the coupling moved population between the components
