@@ -164,6 +164,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->kcolB);
   (void)hipFree(h->twy);
   (void)hipFree(h->cpl);
+  (void)hipFree(h->stage);
   (void)hipFree(h->kcol);
   (void)hipFree(h->kax);
   for (int i = 0; i < 2; ++i) {
@@ -235,15 +236,46 @@ int gpe_set_params(gpe_handle* h, const gpe_params* p) {
   return GPE_OK;
 }
 
-int gpe_set_potential(gpe_handle* h, int comp, const double* V) {
+// C-order host arrays [ny][nx](nz) are copied to a staging buffer and reordered into
+// the column-major device layout on the GPU: (ny, nx, nz) in 3D, (ny, 1, nx) in 2D.
+static int ensure_stage(gpe_handle* h) {
+  if (!h->stage) HIPCHK(h, hipMalloc(&h->stage, h->npts * sizeof(double2)));
+  return GPE_OK;
+}
+
+static int reorder(gpe_handle* h, const void* in, void* out, int elem, bool to_colmajor, double scale) {
+  const unsigned ny = (unsigned)h->ny, mid = h->ndims == 3 ? (unsigned)h->nx : 1u;
+  const unsigned last = h->ndims == 3 ? (unsigned)h->nz : (unsigned)h->nx;
+  HIPCHK(h, gpe::launch_reorder(in, out, elem, ny, mid, last, to_colmajor, scale, h->stream));
+  return GPE_OK;
+}
+
+// Host (column-major or C order) -> device column-major copy of npts values of elem bytes.
+static int upload(gpe_handle* h, void* dst, const void* src, int elem, bool c_order) {
+  int rc;
+  if (!c_order) {
+    HIPCHK(h, hipMemcpyAsync(dst, src, h->npts * elem, hipMemcpyHostToDevice, h->stream));
+    return GPE_OK;
+  }
+  if ((rc = ensure_stage(h))) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->stage, src, h->npts * elem, hipMemcpyHostToDevice, h->stream));
+  return reorder(h, h->stage, dst, elem, true, 1.0);
+}
+
+static int set_potential_impl(gpe_handle* h, int comp, const double* V, bool c_order) {
   if (!h || !V || comp < 0 || comp > 1 || !h->V[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
   HIPCHK(h, hipSetDevice(h->device));
-  HIPCHK(h, hipMemcpyAsync(h->V[comp], V, h->npts * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int rc;
+  if ((rc = upload(h, h->V[comp], V, (int)sizeof(double), c_order))) return rc;
   if (comp == 0) ++h->v_gen;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_V[comp] = true;
   return GPE_OK;
 }
+
+int gpe_set_potential(gpe_handle* h, int comp, const double* V) { return set_potential_impl(h, comp, V, false); }
+
+int gpe_set_potential_c(gpe_handle* h, int comp, const double* V) { return set_potential_impl(h, comp, V, true); }
 
 int gpe_set_potential_mu(gpe_handle* h, int comp, const double* V) {
   if (!h || comp < 0 || comp > 1) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
@@ -259,19 +291,23 @@ int gpe_set_potential_mu(gpe_handle* h, int comp, const double* V) {
   return GPE_OK;
 }
 
-int gpe_set_state(gpe_handle* h, int comp, const double* psi) {
+static int set_state_impl(gpe_handle* h, int comp, const double* psi, bool c_order) {
   if (!h || !psi || comp < 0 || comp > 1 || !h->psi[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
   HIPCHK(h, hipSetDevice(h->device));
   int rc;
   // a pending scale belongs to the other component as well: materialise first
   if (h->prm.ncomp == 2 && (rc = materialize(h))) return rc;
-  HIPCHK(h, hipMemcpyAsync(h->psi[comp], psi, h->npts * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  if ((rc = upload(h, h->psi[comp], psi, (int)sizeof(double2), c_order))) return rc;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
   s.scale = 1.0;
   if ((rc = write_state(h, s))) return rc;
   return GPE_OK;
 }
+
+int gpe_set_state(gpe_handle* h, int comp, const double* psi) { return set_state_impl(h, comp, psi, false); }
+
+int gpe_set_state_c(gpe_handle* h, int comp, const double* psi) { return set_state_impl(h, comp, psi, true); }
 
 int gpe_get_state(gpe_handle* h, int comp, double* psi) {
   if (!h || !psi || comp < 0 || comp > 1 || !h->psi[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
@@ -288,12 +324,28 @@ int gpe_get_state(gpe_handle* h, int comp, double* psi) {
   return GPE_OK;
 }
 
-int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
+int gpe_get_state_c(gpe_handle* h, int comp, double* psi) {
+  if (!h || !psi || comp < 0 || comp > 1 || !h->psi[comp]) return h ? fail(h, GPE_EINVAL, "bad component") : GPE_EINVAL;
+  HIPCHK(h, hipSetDevice(h->device));
+  DevState s;
+  int rc;
+  if ((rc = read_state(h, &s))) return rc;
+  if ((rc = ensure_stage(h))) return rc;
+  // the pending renormalisation (solve_split.m:72) is applied in the reorder, as the
+  // same multiplication gpe_get_state does on the host
+  if ((rc = reorder(h, h->psi[comp], h->stage, (int)sizeof(double2), false, s.scale))) return rc;
+  HIPCHK(h, hipMemcpyAsync(psi, h->stage, h->npts * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return GPE_OK;
+}
+
+static int set_coupling_impl(gpe_handle* h, const double* omega, int is_array, bool c_order) {
   if (!h || !omega) return GPE_EINVAL;
   HIPCHK(h, hipSetDevice(h->device));
   if (is_array) {
+    int rc;
     if (!h->cpl) HIPCHK(h, hipMalloc(&h->cpl, h->npts * sizeof(double2)));
-    HIPCHK(h, hipMemcpyAsync(h->cpl, omega, h->npts * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    if ((rc = upload(h, h->cpl, omega, (int)sizeof(double2), c_order))) return rc;
     double m2 = 0.0;
     for (long long i = 0; i < h->npts; ++i) {
       const double a2 = omega[2 * i] * omega[2 * i] + omega[2 * i + 1] * omega[2 * i + 1];
@@ -308,6 +360,14 @@ int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
     h->cpl_im = omega[1];
   }
   return GPE_OK;
+}
+
+int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array) {
+  return set_coupling_impl(h, omega, is_array, false);
+}
+
+int gpe_set_coupling_c(gpe_handle* h, const double* omega, int is_array) {
+  return set_coupling_impl(h, omega, is_array, true);
 }
 
 int gpe_set_omega(gpe_handle* h, double omega) {

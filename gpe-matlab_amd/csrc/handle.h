@@ -46,6 +46,7 @@ struct gpe_handle {
   double2* psi[2] = {nullptr, nullptr};
   double* V[2] = {nullptr, nullptr};
   double2* cpl = nullptr;       // spinor coupling array (or null for scalar)
+  void* stage = nullptr;        // C-order transfers: npts complex values (allocated on first use)
   double cpl_re = 0, cpl_im = 0;
   bool cpl_array = false;
   double cpl_max2 = 0;          // max |Omega|^2 of the coupling array (NaN propagates)

@@ -57,5 +57,11 @@ void wisdom_clear();
 // every recorded (ndims, axis, n, variant)
 std::vector<std::array<int, 4>> wisdom_entries();
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
+// Array order conversion between C order [ny][nmid][nlast] (last index fastest) and
+// column-major [ny, nmid, nlast] (first index fastest).  to_colmajor: in is C order,
+// out column-major; otherwise the reverse, with every value multiplied by `scale`
+// (skipped when scale == 1).  elem: 16 (complex128) or 8 (float64) bytes.
+hipError_t launch_reorder(const void* in, void* out, int elem, unsigned ny, unsigned nmid, unsigned nlast,
+                          bool to_colmajor, double scale, hipStream_t s);
 
 }  // namespace gpe

@@ -256,6 +256,67 @@ hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* ou
   return hipGetLastError();
 }
 
+__device__ __forceinline__ double scaled(double x, double s) { return x * s; }
+__device__ __forceinline__ double2 scaled(double2 x, double s) { return make_double2(x.x * s, x.y * s); }
+
+// One 32 x 32 tile of the (y, last) matrix of one middle index per work-group
+// (32 x 8 threads): loads run along the input's contiguous index and stores along
+// the output's, through a padded LDS tile.
+template <class T, bool TO_COL>
+__global__ void __launch_bounds__(256) reorder_kernel(const T* __restrict__ in, T* __restrict__ out, unsigned ny,
+                                                      unsigned nmid, unsigned nlast, double scale) {
+  __shared__ T tile[32][33];
+  const unsigned tx = threadIdx.x & 31u, ty = threadIdx.x >> 5;
+  const unsigned z0 = blockIdx.x * 32u, y0 = blockIdx.y * 32u, im = blockIdx.z;
+  const bool sc = scale != 1.0;
+  if constexpr (TO_COL) {
+    for (unsigned r = ty; r < 32u; r += 8u) {   // C order: z fastest
+      const unsigned iy = y0 + r, iz = z0 + tx;
+      if (iy < ny && iz < nlast) tile[r][tx] = in[((size_t)iy * nmid + im) * nlast + iz];
+    }
+    __syncthreads();
+    for (unsigned r = ty; r < 32u; r += 8u) {   // column-major: y fastest
+      const unsigned iz = z0 + r, iy = y0 + tx;
+      if (iy < ny && iz < nlast) out[iy + (size_t)ny * (im + (size_t)nmid * iz)] = tile[tx][r];
+    }
+  } else {
+    for (unsigned r = ty; r < 32u; r += 8u) {
+      const unsigned iz = z0 + r, iy = y0 + tx;
+      if (iy < ny && iz < nlast) tile[tx][r] = in[iy + (size_t)ny * (im + (size_t)nmid * iz)];
+    }
+    __syncthreads();
+    for (unsigned r = ty; r < 32u; r += 8u) {
+      const unsigned iy = y0 + r, iz = z0 + tx;
+      if (iy < ny && iz < nlast) {
+        const T v = tile[r][tx];
+        out[((size_t)iy * nmid + im) * nlast + iz] = sc ? scaled(v, scale) : v;
+      }
+    }
+  }
+}
+
+hipError_t launch_reorder(const void* in, void* out, int elem, unsigned ny, unsigned nmid, unsigned nlast,
+                          bool to_colmajor, double scale, hipStream_t s) {
+  if (ny == 0u || nmid == 0u || nlast == 0u || nmid > 65535u || (elem != 8 && elem != 16)) return hipErrorInvalidValue;
+  const dim3 grid((nlast + 31u) / 32u, (ny + 31u) / 32u, nmid), block(256);
+  if (elem == 16) {
+    const double2* i = static_cast<const double2*>(in);
+    double2* o = static_cast<double2*>(out);
+    if (to_colmajor)
+      hipLaunchKernelGGL((reorder_kernel<double2, true>), grid, block, 0, s, i, o, ny, nmid, nlast, scale);
+    else
+      hipLaunchKernelGGL((reorder_kernel<double2, false>), grid, block, 0, s, i, o, ny, nmid, nlast, scale);
+  } else {
+    const double* i = static_cast<const double*>(in);
+    double* o = static_cast<double*>(out);
+    if (to_colmajor)
+      hipLaunchKernelGGL((reorder_kernel<double, true>), grid, block, 0, s, i, o, ny, nmid, nlast, scale);
+    else
+      hipLaunchKernelGGL((reorder_kernel<double, false>), grid, block, 0, s, i, o, ny, nmid, nlast, scale);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipStream_t s, int* nblocks, int var) {
   switch (n) {
 #define GPE_C(N) case N: return launch_sweep_##N(seq, colfast, var, p, s, nblocks);

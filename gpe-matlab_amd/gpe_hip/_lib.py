@@ -67,6 +67,10 @@ SIGNATURES = {
     "gpe_set_state": [_P, C.c_int, _DP],
     "gpe_get_state": [_P, C.c_int, _DP],
     "gpe_set_coupling": [_P, _DP, C.c_int],
+    "gpe_set_potential_c": [_P, C.c_int, _DP],
+    "gpe_set_state_c": [_P, C.c_int, _DP],
+    "gpe_get_state_c": [_P, C.c_int, _DP],
+    "gpe_set_coupling_c": [_P, _DP, C.c_int],
     "gpe_set_omega": [_P, _D],
     "gpe_sgpe_begin": [_P, C.POINTER(SgpeArgs)],
     "gpe_sgpe_block": [_P, C.c_longlong, _DP],
@@ -167,6 +171,17 @@ def from_colmajor_c128(flat, shape):
     return flat.reshape(tuple(reversed(shape))).T.copy(order="C")
 
 
+def host_order(a, shape, dtype):
+    """(flat array, c_order) for a host array of a grid's shape.  A column-major array
+    (MATLAB data, np.asfortranarray) goes as it is to the column-major entry points;
+    any other goes in C order (copied only when it is not C-contiguous already) to the
+    entry points that reorder on the device."""
+    a = np.broadcast_to(np.asarray(a, dtype=dtype), shape)
+    if a.flags.f_contiguous and not a.flags.c_contiguous:
+        return a.T.reshape(-1), False
+    return np.ascontiguousarray(a).reshape(-1), True
+
+
 class Device:
     """Owns one gpe_handle (device buffers, tables, stream) for one grid."""
 
@@ -235,25 +250,28 @@ class Device:
         self.call("gpe_set_params", C.byref(p))
 
     def set_potential(self, comp, V):
-        v = to_colmajor_f64(np.broadcast_to(V, self.shape))
-        self.call("gpe_set_potential", comp, dptr(v))
+        v, c_order = host_order(V, self.shape, np.float64)
+        self.call("gpe_set_potential_c" if c_order else "gpe_set_potential", comp, v.ctypes.data_as(_DP))
 
     def set_state(self, comp, psi):
-        v = to_colmajor_c128(np.broadcast_to(psi, self.shape))
-        self.call("gpe_set_state", comp, v.view(np.float64).ctypes.data_as(_DP))
+        v, c_order = host_order(psi, self.shape, np.complex128)
+        self.call("gpe_set_state_c" if c_order else "gpe_set_state", comp,
+                  v.view(np.float64).ctypes.data_as(_DP))
 
     def get_state(self, comp=0):
-        out = np.empty(self.npts, dtype=np.complex128)
-        self.call("gpe_get_state", comp, out.view(np.float64).ctypes.data_as(_DP))
-        return from_colmajor_c128(out, self.shape)
+        """The state as a C-order array of the grid's shape (reordered on the device)."""
+        out = np.empty(self.shape, dtype=np.complex128)
+        self.call("gpe_get_state_c", comp, out.view(np.float64).ctypes.data_as(_DP))
+        return out
 
     def set_coupling(self, omega):
         if np.isscalar(omega) or np.ndim(omega) == 0:
             v = np.array([np.real(omega), np.imag(omega)], dtype=np.float64)
             self.call("gpe_set_coupling", dptr(v), 0)
         else:
-            v = to_colmajor_c128(np.broadcast_to(omega, self.shape))
-            self.call("gpe_set_coupling", v.view(np.float64).ctypes.data_as(_DP), 1)
+            v, c_order = host_order(omega, self.shape, np.complex128)
+            self.call("gpe_set_coupling_c" if c_order else "gpe_set_coupling",
+                      v.view(np.float64).ctypes.data_as(_DP), 1)
 
     def vtd_set(self, basis, vtraps):
         G = np.concatenate([to_colmajor_f64(np.broadcast_to(b, self.shape)) for b in basis])

@@ -80,6 +80,18 @@ int gpe_set_potential_mu(gpe_handle* h, int comp, const double* V);
 int gpe_set_state(gpe_handle* h, int comp, const double* psi);
 int gpe_get_state(gpe_handle* h, int comp, double* psi);
 
+/* The same transfers for arrays in C order, [ny][nx] (2D) or [ny][nx][nz] (3D) with
+ * the last index fastest, as numpy and C callers hold them.  No reference
+ * counterpart: MATLAB arrays are column-major and use the functions above.  The array
+ * is copied to a device staging buffer (npts complex values, allocated on first use
+ * and kept until gpe_destroy) and reordered on the GPU, so the caller needs no
+ * host-side transpose.  Results are identical to the column-major functions applied
+ * to the transposed array. */
+int gpe_set_state_c(gpe_handle* h, int comp, const double* psi);
+int gpe_get_state_c(gpe_handle* h, int comp, double* psi);
+int gpe_set_potential_c(gpe_handle* h, int comp, const double* V);
+int gpe_set_coupling_c(gpe_handle* h, const double* omega, int is_array);
+
 /* Rotation frequency of the rotating frame (GPEtask.m:7, scalar omega): the kinetic
  * step becomes the axis-split step of set_kinop.m:5-7 / ssft_kin_step.m:22-25 and
  * applyham's kinetic term lap(phi, omega) (grid2d/lap.m, grid3d/lap.m).  One
