@@ -676,6 +676,11 @@ int gpe_batch_tune(gpe_batch* b, int remeasure, double* times_us, int cap, int* 
     // the sweeps run in place: keep every trajectory's state and put it back afterwards
     if (chk(hipMalloc(&save, b->total * sizeof(double2)), "save") &&
         chk(hipMemcpyAsync(save, b->psi, b->total * sizeof(double2), hipMemcpyDeviceToDevice, b->stream), "save") &&
+        // timed on a synthetic field of unit modulus and, before gpe_batch_split_begin
+        // has built them, unit-modulus kinetic tables (all-zero buffers run faster)
+        chk(gpe::launch_tune_fill(b->psi, b->total, nullptr, 0, b->stream), "fill") &&
+        (b->begun || (chk(gpe::launch_tune_fill(b->kcol, b->ktotal_x, nullptr, 0, b->stream), "fill") &&
+                      chk(gpe::launch_tune_fill(b->kax, b->ktotal_y, nullptr, 0, b->stream), "fill"))) &&
         chk(hipMalloc(&items, maxitems * sizeof(gpe::WorkItem)), "items") && chk(hipEventCreate(&e0), "event") &&
         chk(hipEventCreate(&e1), "event")) {
       for (size_t pi : todo) {

@@ -57,6 +57,9 @@ void wisdom_clear();
 // every recorded (ndims, axis, n, variant)
 std::vector<std::array<int, 4>> wisdom_entries();
 hipError_t launch_phase_selftest(const double* er, const double* ei, double2* out, int n, hipStream_t s);
+// Synthetic inputs for the plan tuners: z[0..nz) of unit modulus with a hashed phase
+// and, when v is not null, v[0..nv) hashed in [0, 10).
+hipError_t launch_tune_fill(double2* z, long long nz, double* v, long long nv, hipStream_t s);
 // Array order conversion between C order [ny][nmid][nlast] (last index fastest) and
 // column-major [ny, nmid, nlast] (first index fastest).  to_colmajor: in is C order,
 // out column-major; otherwise the reverse, with every value multiplied by `scale`

@@ -256,6 +256,37 @@ hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* ou
   return hipGetLastError();
 }
 
+// splitmix64 finaliser of the index: a uniform double in [0, 1)
+__device__ __forceinline__ double hash_unit(unsigned long long i) {
+  unsigned long long x = i * 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * 0x1.0p-53;
+}
+
+__global__ void __launch_bounds__(256) tune_fill_kernel(double2* z, long long nz, double* v, long long nv) {
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nz || i < nv; i += stride) {
+    const double u = hash_unit((unsigned long long)i);
+    if (i < nz) {
+      double sn, cs;
+      sincospi(2.0 * u, &sn, &cs);
+      z[i] = make_double2(cs, sn);
+    }
+    if (v && i < nv) v[i] = 10.0 * u;
+  }
+}
+
+hipError_t launch_tune_fill(double2* z, long long nz, double* v, long long nv, hipStream_t s) {
+  const long long n = nz > nv ? nz : nv;
+  if (n <= 0) return hipSuccess;
+  const long long want = (n + 255) / 256;
+  const unsigned blocks = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(tune_fill_kernel, dim3(blocks), dim3(256), 0, s, z, nz, v, v ? nv : 0ll);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ double scaled(double x, double s) { return x * s; }
 __device__ __forceinline__ double2 scaled(double2 x, double s) { return make_double2(x.x * s, x.y * s); }
 
