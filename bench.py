@@ -17,6 +17,8 @@ Legs:
   profile      one outer iteration with HIP events around every sweep launch
                (on the library's stream) -> per-kernel average durations
   cpu_baseline the oracle's steps/s on this host (the same 10 steps)
+  host_inclusive  the same steps as one drop-in solve_split call from host arrays
+               (new task, uploads, graph capture, download of the final state)
 """
 import argparse
 import json
@@ -77,6 +79,8 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c5: skip the Rabi adjoint leg")
     ap.add_argument("--no-tune", action="store_true", help="c4: keep the default FFT plans (no plan tuner)")
+    ap.add_argument("--no-per-trajectory", action="store_true",
+                    help="c4: skip the comparison leg with one handle per trajectory")
     ap.add_argument("--ysplit", action="store_true",
                     help="c2: run the two-sweep step (256-point y transform split 16 x 16, GPE_YSPLIT=1)")
     ap.add_argument("--omega", type=float, default=0.0,
@@ -427,7 +431,24 @@ def bench_c4(args):
     el = time.perf_counter() - t0
     ens_steps = nout * ni
     traj_rate = len(ens) * ens_steps / el
-    # per-trajectory launches: one handle per trajectory, the same steps
+    gbps = 72 * npts * (ens_steps / el) / 1e9
+    extra = {}
+    if args.no_per_trajectory:
+        log(f"c4: {len(ens)} trajectories, {npts} points; batched {traj_rate:.0f} traj-steps/s")
+    else:
+        extra = _c4_per_trajectory(ens, ddt, ni, nout, traj_rate, b, log)
+    _line("trajectory-steps/s (256 ragged 2D trajectories)", round(traj_rate, 1), "trajectory-steps/s", ens_steps, ni,
+          el, True, {"workload": "c4: 256 x grid2d(15,N,15,N), N in {128,192,256,384}, g~U[0.01,0.1], "
+                                 "V=0.5(X^2+lam^2 Y^2) lam~U[1,2], TF + vortex |r0|<5, dt=0.004, graph per 100 steps",
+                     "points": npts},
+          plan_tuner_us={f"{'xy'[a]}{n}": [round(t0, 1), round(t1, 1)] for (a, n), (t0, t1) in sorted(tune_us.items())},
+          ensemble_alg_GBps=round(gbps, 1), **extra)
+
+
+def _c4_per_trajectory(ens, ddt, ni, nout, traj_rate, b, log):
+    """The same steps with one handle per trajectory (launch-bound comparison)."""
+    from gpe_hip import _lib
+    ens_steps = nout * ni
     devs = []
     for gr, V, psi, pr in ens:
         d = _lib.Device(gr)
@@ -447,16 +468,11 @@ def bench_c4(args):
     el1 = time.perf_counter() - t0
     single_rate = len(ens) * ens_steps / el1
     same = all(np.array_equal(b.get_state(i), devs[i].get_state(0)) for i in range(0, len(ens), 37))
-    gbps = 72 * npts * (ens_steps / el) / 1e9
-    log(f"c4: {len(ens)} trajectories, {npts} points; batched {traj_rate:.0f} traj-steps/s, per-trajectory "
+    log(f"c4: {len(ens)} trajectories; batched {traj_rate:.0f} traj-steps/s, per-trajectory "
         f"{single_rate:.0f} traj-steps/s, ratio {traj_rate / single_rate:.2f}; bit-identical sample {same}")
-    _line("trajectory-steps/s (256 ragged 2D trajectories)", round(traj_rate, 1), "trajectory-steps/s", ens_steps, ni,
-          el, True, {"workload": "c4: 256 x grid2d(15,N,15,N), N in {128,192,256,384}, g~U[0.01,0.1], "
-                                 "V=0.5(X^2+lam^2 Y^2) lam~U[1,2], TF + vortex |r0|<5, dt=0.004, graph per 100 steps",
-                     "points": npts},
-          per_trajectory_launch_rate=round(single_rate, 1), batched_vs_per_trajectory=round(traj_rate / single_rate, 2),
-          plan_tuner_us={f"{'xy'[a]}{n}": [round(t0, 1), round(t1, 1)] for (a, n), (t0, t1) in sorted(tune_us.items())},
-          ensemble_alg_GBps=round(gbps, 1), batched_equals_single_bitwise=bool(same))
+    return {"per_trajectory_launch_rate": round(single_rate, 1),
+            "batched_vs_per_trajectory": round(traj_rate / single_rate, 2),
+            "batched_equals_single_bitwise": bool(same)}
 
 
 def bench_c5(args):
