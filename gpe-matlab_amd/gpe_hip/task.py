@@ -92,6 +92,13 @@ class GPEtask:
     def _upload(self, phi):
         self._device().set_state(0, phi)
 
+    def _split_result(self, start, niter_outer):
+        # solve_split.m:34-38, 84: phi starts as current_state (resumed) or init_state
+        # and is returned as it stands when the outer loop runs no iteration
+        if start == 0 and niter_outer <= start:
+            return self.init_state
+        return self.current_state
+
     def getVtotal(self, time):
         # GPEtask.m:62-71
         if time == 0 and np.size(self.V0) > 1:
@@ -295,7 +302,7 @@ class GPEtask:
                         dev.set_potential(0, self.getVtotal(time2))   # 54
                     muc, ncur = dev.split_block(time2)
             self.ext_callback(j, time2, muc, ncur)              # 80
-        return self.current_state
+        return self._split_result(start, niter_outer)
 
     def groundstate_itp(self, dt, eps, phi0="rand", nargout=1):
         """groundstate_itp.m:1-112 on the GPU.  ``nargout`` plays MATLAB's nargout."""
@@ -387,7 +394,7 @@ class SGPEtask(GPEtask):
             err = self.ext_callback(j, time2, mu, ncur)         # 77-80
             if err == "TERM":
                 break
-        return self.current_state
+        return self._split_result(start, niter_outer)
 
 
 class GPEtaskSpinor(GPEtask):
@@ -533,7 +540,7 @@ class GPEtaskSpinor(GPEtask):
         finally:
             if not static:
                 dev.set_potential_mu(0, None)
-        return self.current_state
+        return self._split_result(start, niter_outer)
 
     # two-component imaginary time ------------------------------------------------
     itp_max_iter_2c = 500000   # the two-component solvers stop at 500000 iterations

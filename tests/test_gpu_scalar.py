@@ -101,6 +101,22 @@ def test_solve_split_options_parity():
         assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
 
 
+def test_solve_split_without_outer_iterations_matches_oracle():
+    """solve_split.m:34-38, 49, 84: with niter_outer <= current_iter the loop runs no
+    iteration and phi is returned as it started: init_state on a fresh task, the
+    current state on a resumed one (history untouched)."""
+    tg, to = _tasks(2, (32, 24), (7, 6), harm2, gamma=0.05)
+    pg, po = tg.solve_split(0.004, 10, 0), to.solve_split(0.004, 10, 0)
+    assert np.array_equal(pg, po) and np.array_equal(pg, to.init_state)
+    assert tg.current_iter == to.current_iter == 0
+    tg.solve_split(0.004, 10, 2)
+    to.solve_split(0.004, 10, 2)
+    pg, po = tg.solve_split(0.004, 10, 2), to.solve_split(0.004, 10, 2)   # nothing left to run
+    assert rel_l2(pg, po) < TOL
+    assert tg.current_iter == to.current_iter == 2
+    assert len(tg.history["mu"]) == len(to.history["mu"]) == 2
+
+
 def test_solve_split_time_dependent_potential_parity():
     """Vtd evaluated per block at time2 (GPEtask.m:65-67, solve_split.m:54)."""
     def vtd(task, t):
