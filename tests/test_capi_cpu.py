@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -107,3 +108,24 @@ def test_plan_wisdom_set_export_import_roundtrip(lib):
     assert lib.gpe_wisdom_import(text.encode()) == 0
     assert lib.gpe_wisdom_get(3, 0, 256) == 1 and lib.gpe_wisdom_get(2, 1, 384) == 1
     assert lib.gpe_wisdom_forget() == 0
+
+
+def test_host_order_picks_entry_point_without_copies():
+    """Device.set_state / set_potential / set_coupling send a column-major array to the
+    column-major entry points as it is, and any other array in C order to the
+    entry points that reorder on the device, copying only when the array is neither."""
+    from gpe_hip import _lib
+    shape = (6, 4, 8)
+    a = (np.arange(np.prod(shape)).reshape(shape) * (1 + 0.5j))
+    v, c = _lib.host_order(a, shape, np.complex128)
+    assert c and np.shares_memory(v, a) and np.array_equal(v.reshape(shape), a)
+    f = np.asfortranarray(a)
+    v, c = _lib.host_order(f, shape, np.complex128)
+    assert not c and np.shares_memory(v, f) and np.array_equal(v, _lib.to_colmajor_c128(a))
+    s = a[:, ::-1]
+    v, c = _lib.host_order(s, shape, np.complex128)
+    assert c and v.flags.c_contiguous and np.array_equal(v.reshape(shape), s)
+    v, c = _lib.host_order(2.5, shape, np.float64)          # scalars broadcast to the grid
+    assert c and v.shape == (np.prod(shape),) and np.all(v == 2.5)
+    v, c = _lib.host_order(a.real, shape, np.complex128)    # real input converted once
+    assert c and v.dtype == np.complex128 and np.array_equal(v.reshape(shape), a.real)
