@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) batch_sweep_kernel(const BatchLaunch b) {
     p.stride = 1;
     p.slot_ke = SL_KY;
   }
-  sweep_body<N, C, COLFAST, SEQ, VAR>(p, (unsigned)it.tile, (unsigned)it.tile, lds, twl, red);
+  sweep_body<N, C, COLFAST, SEQ, VAR>(p, p.psi, (unsigned)it.tile, (unsigned)it.tile, lds, twl, red);
 }
 
 #define GPE_CAT2(a, b) a##b

@@ -131,6 +131,29 @@ int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax
   return GPE_OK;
 }
 
+// spinor_rest_of_step for both components, one launch per sweep: the second
+// component runs on the second half of each grid (same operations per component).
+static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) {
+  int rc;
+  const int ka = h->ndims == 3 ? 2 : 1;
+  if (h->ndims == 3) {
+    SweepParams f = base_params(h, 1, h->psi[0], nullptr);
+    f.psi2 = h->psi[1];
+    if ((rc = sweep(h, 1, gpe::S_FWD, f))) return rc;
+  }
+  SweepParams k = base_params(h, ka, h->psi[0], nullptr);
+  k.psi2 = h->psi[1];
+  k.kcol = kcol;
+  k.kax = kax;
+  if ((rc = sweep(h, ka, gpe::S_KIN, k))) return rc;
+  if (h->ndims == 3) {
+    SweepParams v = base_params(h, 1, h->psi[0], nullptr);
+    v.psi2 = h->psi[1];
+    if ((rc = sweep(h, 1, gpe::S_INV, v))) return rc;
+  }
+  return GPE_OK;
+}
+
 int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
   int rc;
   // open: x forward (with the pending renormalisation), then exp(-kk dt/2)
@@ -138,8 +161,8 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
     SweepParams p = base_params(h, 0, h->psi[comp], nullptr);
     p.apply_scale = 1;
     if ((rc = sweep(h, 0, gpe::S_FWD, p))) return rc;
-    if ((rc = spinor_rest_of_step(h, h->psi[comp], h->kcol_sp[0], h->kax_sp[0]))) return rc;
   }
+  if ((rc = spinor_rest_of_step_both(h, h->kcol_sp[0], h->kax_sp[0]))) return rc;
   for (int i = 0; i < h->n_rec; ++i) {
     gpe::SpinorParams p = spinor_params(h);
     if (tape) {
@@ -148,9 +171,7 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
     }
     if ((rc = spinor_sweep(h, gpe::SP_PR, p, nullptr))) return rc;
     const bool last = i + 1 == h->n_rec;
-    for (int comp = 0; comp < 2; ++comp)
-      if ((rc = spinor_rest_of_step(h, h->psi[comp], last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax)))
-        return rc;
+    if ((rc = spinor_rest_of_step_both(h, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax))) return rc;
   }
   for (int comp = 0; comp < 2; ++comp)
     if ((rc = sweep(h, 0, gpe::S_INV, base_params(h, 0, h->psi[comp], nullptr)))) return rc;

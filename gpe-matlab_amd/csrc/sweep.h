@@ -111,6 +111,9 @@ struct SweepParams {
   // y-split 3D steps (S_Y*)
   const double2* twy;        // y twiddles e^{-2 pi i j/256}
   unsigned gnx, gny, gnz;    // grid sizes (x sweeps of S_YOPEN/S_YMID/S_YCLOSEM address the grid directly)
+  // S_FWD / S_INV / S_KIN of sweep_kernel: a second array of the same shape, swept by
+  // the second half of the grid with the same tables (null: one array)
+  double2* psi2;
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

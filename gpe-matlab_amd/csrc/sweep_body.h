@@ -74,8 +74,8 @@ __device__ __forceinline__ double ke_weight(const SweepParams& p, int i, double 
 // S_KIN per-index kinetic factors), `red` 4 doubles;
 // reductions go to partials[slot * maxwg + wg].
 template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
-__device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, unsigned wg, double* lds,
-                                           double2* twl, double* red) {
+__device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __restrict__ psi, unsigned tile,
+                                           unsigned wg, double* lds, double2* twl, double* red) {
   using PL = typename SweepPlan<N, SEQ, VAR>::type;
   using TR = SeqTraits<SEQ>;
   constexpr int T = PL::T, E = PL::E;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
 
   double2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = p.psi[base + (unsigned)m * step];
+  for (int m = 0; m < E; ++m) v[m] = psi[base + (unsigned)m * step];
   double vv[TR::LOAD_V ? E : 1];
   if constexpr (TR::LOAD_V) {
 #pragma unroll
@@ -287,7 +287,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, unsigned tile, 
   if constexpr (TR::STORE) {
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psi[base + (unsigned)m * step] = v[m];
     }
   }
 
@@ -347,9 +347,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN ? 2 * N : N];
   __shared__ double red[4];
-  const unsigned b = C < 16 ? xcd_pair_tile(blockIdx.x, gridDim.x) : blockIdx.x;
-  const unsigned tile = p.reverse ? gridDim.x - 1u - b : b;
-  sweep_body<N, C, COLFAST, SEQ, VAR>(p, tile, tile, lds, twl, red);
+  unsigned bid = blockIdx.x, nbk = gridDim.x;
+  double2* psi = p.psi;
+  if (p.psi2) {   // two arrays: the second half of the grid sweeps psi2 (no reductions)
+    nbk >>= 1;
+    if (bid >= nbk) {
+      bid -= nbk;
+      psi = p.psi2;
+    }
+  }
+  const unsigned b = C < 16 ? xcd_pair_tile(bid, nbk) : bid;
+  const unsigned tile = p.reverse ? nbk - 1u - b : b;
+  sweep_body<N, C, COLFAST, SEQ, VAR>(p, psi, tile, tile, lds, twl, red);
 }
 
 }  // namespace gpe

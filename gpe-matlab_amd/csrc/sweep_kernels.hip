@@ -19,10 +19,11 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, int var, const S
   // pencils per work-group follow the sequence's plan (sweep_cols); variant 1 exists
   // for the tunable sequences of lengths with a second plan
   if (!has_variant<N>() || !tunable_seq(seq)) var = 0;
+  if (p.psi2 && seq != S_FWD && seq != S_INV && seq != S_KIN) return hipErrorInvalidValue;
 #define GPE_LV(CF, S, V)                                                                               \
   {                                                                                                    \
     constexpr int C = sweep_cols<N, S, V>();                                                           \
-    nb = ((long long)p.ncols + C - 1) / C;                                                             \
+    nb = ((long long)p.ncols + C - 1) / C * (p.psi2 ? 2 : 1);                                          \
     if (nb > 0) hipLaunchKernelGGL((sweep_kernel<N, C, CF, S, V>), dim3((unsigned)nb), block, 0, s, p); \
   }
 #define GPE_L(CF, S) \
