@@ -156,9 +156,11 @@ static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) 
 
 int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
   int rc;
-  // open: x forward (with the pending renormalisation), then exp(-kk dt/2)
-  for (int comp = 0; comp < 2; ++comp) {
-    SweepParams p = base_params(h, 0, h->psi[comp], nullptr);
+  // open: x forward of both components (with the pending renormalisation), then
+  // exp(-kk dt/2)
+  {
+    SweepParams p = base_params(h, 0, h->psi[0], nullptr);
+    p.psi2 = h->psi[1];
     p.apply_scale = 1;
     if ((rc = sweep(h, 0, gpe::S_FWD, p))) return rc;
   }
@@ -173,8 +175,11 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
     const bool last = i + 1 == h->n_rec;
     if ((rc = spinor_rest_of_step_both(h, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax))) return rc;
   }
-  for (int comp = 0; comp < 2; ++comp)
-    if ((rc = sweep(h, 0, gpe::S_INV, base_params(h, 0, h->psi[comp], nullptr)))) return rc;
+  {
+    SweepParams p = base_params(h, 0, h->psi[0], nullptr);
+    p.psi2 = h->psi[1];
+    if ((rc = sweep(h, 0, gpe::S_INV, p))) return rc;
+  }
   if (!reduce) return GPE_OK;
   FinParams f = fin_params(h, gpe::FIN_SPINOR);
   if ((rc = spinor_reduce(h, &f, false))) return rc;
