@@ -14,8 +14,9 @@ Legs:
   parity       the GPU and the FP64 oracle run the same first 10 steps from the
                same state; rel-L2 of the GPU state against the oracle
   timed        warmup, then --steps steps bracketed by device synchronisation
-  profile      one outer iteration with HIP events around every sweep launch
-               (on the library's stream) -> per-kernel average durations
+               (both counts rounded up to whole 10-step blocks: n_recalc = 10)
+  profile      the timed outer iterations again with HIP events around every
+               sweep launch (on the library's stream) -> per-kernel average durations
   cpu_baseline the oracle's steps/s on this host (the same 10 steps)
   host_inclusive  the same steps as one drop-in solve_split call from host arrays
                (new task, uploads, graph capture, download of the final state)
@@ -53,6 +54,24 @@ def committed_traffic(n):
         if line.startswith("traffic_bytes_per_launch:"):
             return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].strip().split(", "))}
     return {}
+
+
+def plan_steps(steps, warmup, n_recalc=10):
+    """Outer-iteration schedule for the timed and warm-up regions.
+
+    A step count that is not a multiple of n_recalc would make solve_split fall back
+    to n_rec = 1 (solve_split.m:25-27), a different workload, so both counts are
+    rounded up to whole n_recalc blocks (the warm-up to at least one, which captures
+    the outer-iteration graph).  Outer iterations are 50 steps when both
+    rounded counts allow it, else n_recalc steps.  Returns (ni, warm_outer,
+    timed_outer); the steps actually timed are ni * timed_outer >= steps.
+    """
+    steps = max(1, int(steps))
+    warmup = max(0, int(warmup))
+    s = -(-steps // n_recalc) * n_recalc
+    w = max(n_recalc, -(-warmup // n_recalc) * n_recalc)   # at least one block: graph capture
+    ni = 50 if (s % 50 == 0 and w % 50 == 0) else n_recalc
+    return ni, w // ni, s // ni
 
 
 def sync():
@@ -124,9 +143,11 @@ def main():
     from gpe_hip import workloads
     from gpe_hip._lib import AXIS_NAMES, SEQ_NAMES
 
-    ni = 50 if (args.steps % 50 == 0 and args.warmup % 50 == 0) else 10
-    if args.steps % ni or args.warmup % ni:
-        sys.exit("--steps and --warmup must be multiples of 10")
+    ni, warm_outer, timed_outer = plan_steps(args.steps, args.warmup)
+    steps_timed, warm_run = ni * timed_outer, ni * warm_outer
+    if steps_timed != args.steps or warm_run != args.warmup:
+        log(f"--steps {args.steps} --warmup {args.warmup} rounded up to whole 10-step blocks: "
+            f"{steps_timed} timed, {warm_run} warm-up steps ({ni} per outer iteration)")
     ddt = 0.002
     t_setup = time.perf_counter()
     grid, V, psi0, props = workloads.c2_state(n=args.n)
@@ -161,57 +182,31 @@ def main():
             if (j * 10) % every == 0:
                 gpu_ck[j * 10] = dev.get_state(0)
         log(f"parity leg: {cs} GPU steps done")
-        import oracle
-        from oracle import grids as ogrids
-        O = oracle.grid3d(grid.xmax, grid.nx, grid.ymax, grid.ny, grid.zmax, grid.nz)
-        to = oracle.GPEtask(O, V)
-        to.Ntotal, to.g, to.gamma = props["Ntotal"], props["g"], props["gamma"]
-        to.omega = args.omega
-        to.init_state = psi0
-        cpu_ck = {}
-
-        t_start = time.perf_counter()
-
-        def grab(t):
-            if (t.current_iter * 10) % every == 0:
-                cpu_ck[t.current_iter * 10] = t.current_state.copy()
-                log(f"  oracle at step {t.current_iter * 10} ({time.perf_counter() - t_start:.0f}s)")
-
-        to.user_callback = grab
-        t0 = time.perf_counter()
-        to.solve_split(ddt, 10, cs // 10)
-        el = time.perf_counter() - t0
-        cpu_baseline = {"value": cs / el, "unit": "split-steps/s", "cores": ogrids.WORKERS, "kind": "port",
-                        "sample": f"FP64 numpy/scipy restatement of solve_split.m, {cs} steps of the same 256^3 "
-                                  f"workload in {el:.1f}s; scipy.fft workers={ogrids.WORKERS}, "
-                                  "elementwise passes single-threaded"}
-        for k in sorted(gpu_ck):
-            e = float(np.linalg.norm(gpu_ck[k] - cpu_ck[k]) / np.linalg.norm(cpu_ck[k]))
-            parity_curve.append([k, e])
-        parity = parity_curve[-1][1]
-        log(f"cpu leg: {cs} oracle steps in {el:.1f}s; GPU rel-L2 vs oracle {parity_curve}")
 
     # ---------------------------------------------------------------- timed leg
     dev.set_state(0, psi0)
     dev.split_begin(ddt, ni)
     j = 0
-    for _ in range(args.warmup // ni):
+    for _ in range(warm_outer):
         j += 1
         dev.split_outer(j)
     sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps // ni):
+    for _ in range(timed_outer):
         j += 1
         _, muc, ncur = dev.split_outer(j)
     sync()
     el = time.perf_counter() - t0
-    steps_per_s = args.steps / el
-    log(f"timed: {args.steps} steps in {el:.3f}s -> {steps_per_s:.1f} steps/s (mu {muc:.6f}, n {ncur:.6f})")
+    steps_per_s = steps_timed / el
+    log(f"timed: {steps_timed} steps in {el:.4f}s -> {steps_per_s:.1f} steps/s (mu {muc:.6f}, n {ncur:.6f})")
 
     # ---------------------------------------------------------------- profile leg
+    # The timed region's outer iterations once more, launched directly with HIP events
+    # recorded around every sweep on the library's stream.
     dev.profile_enable(True)
-    j += 1
-    dev.split_outer(j)
+    for _ in range(timed_outer):
+        j += 1
+        dev.split_outer(j)
     prof = dev.profile_read()
     dev.profile_enable(False)
     kernels = []
@@ -244,25 +239,60 @@ def main():
     t2.Ntotal, t2.g, t2.gamma = props["Ntotal"], props["g"], props["gamma"]
     t2.omega = args.omega
     t2.init_state = psi0
-    phi = t2.solve_split(ddt, ni, args.steps // ni)
+    phi = t2.solve_split(ddt, ni, timed_outer)
     sync()
     el_host = time.perf_counter() - t0
-    host_inclusive = {"steps_per_s": round(args.steps / el_host, 2), "steps": args.steps,
+    host_inclusive = {"steps_per_s": round(steps_timed / el_host, 2), "steps": steps_timed,
                       "seconds": round(el_host, 4), "final_state_finite": bool(np.isfinite(phi).all()),
                       "includes": "new task, upload of psi0 (256 MiB) and V (128 MiB), kinetic tables, "
                                   "graph capture, download of the final state"}
     del t2, phi
-    log(f"host-inclusive drop-in call: {args.steps} steps in {el_host:.3f}s -> "
+    log(f"host-inclusive drop-in call: {steps_timed} steps in {el_host:.3f}s -> "
         f"{host_inclusive['steps_per_s']:.1f} steps/s")
+
+    # ---------------------------------------------------------------- oracle leg
+    # The oracle runs after every GPU leg: its seconds of host work would otherwise leave
+    # the GPU idle (and its clocks low) right before the timed region.
+    if not (args.no_parity and args.no_cpu_baseline):
+        import oracle
+        from oracle import grids as ogrids
+        O = oracle.grid3d(grid.xmax, grid.nx, grid.ymax, grid.ny, grid.zmax, grid.nz)
+        to = oracle.GPEtask(O, V)
+        to.Ntotal, to.g, to.gamma = props["Ntotal"], props["g"], props["gamma"]
+        to.omega = args.omega
+        to.init_state = psi0
+        cpu_ck = {}
+
+        t_start = time.perf_counter()
+
+        def grab(t):
+            if (t.current_iter * 10) % every == 0:
+                cpu_ck[t.current_iter * 10] = t.current_state.copy()
+                log(f"  oracle at step {t.current_iter * 10} ({time.perf_counter() - t_start:.0f}s)")
+
+        to.user_callback = grab
+        t0 = time.perf_counter()
+        to.solve_split(ddt, 10, cs // 10)
+        el_cpu = time.perf_counter() - t0
+        cpu_baseline = {"value": cs / el_cpu, "unit": "split-steps/s", "cores": ogrids.WORKERS, "kind": "port",
+                        "sample": f"FP64 numpy/scipy restatement of solve_split.m, {cs} steps of the same 256^3 "
+                                  f"workload in {el_cpu:.1f}s; scipy.fft workers={ogrids.WORKERS}, "
+                                  "elementwise passes single-threaded"}
+        for k in sorted(gpu_ck):
+            e = float(np.linalg.norm(gpu_ck[k] - cpu_ck[k]) / np.linalg.norm(cpu_ck[k]))
+            parity_curve.append([k, e])
+        parity = parity_curve[-1][1]
+        log(f"cpu leg: {cs} oracle steps in {el_cpu:.1f}s; GPU rel-L2 vs oracle {parity_curve}")
 
     out = {
         "metric": "split-steps/s (GPEtask.solve_split, 256^3 complex128)",
         "value": round(steps_per_s, 2),
         "unit": "split-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / args.steps, 4),
+        "steps": steps_timed,
+        "warmup": warm_run,
+        "steps_requested": [args.steps, args.warmup],
+        "ms_per_step": round(1e3 * el / steps_timed, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
