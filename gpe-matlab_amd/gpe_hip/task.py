@@ -509,14 +509,20 @@ class GPEtaskSpinor(GPEtask):
         n_rec = self.n_recalc
         dt_outer = ddt * niter_inner
         cpl = self.coupling
+        # what was uploaded: a callback may rebind task.coupling or change an array
+        # coupling in place (task.coupling *= 2), and MATLAB's value semantics make both
+        # visible at the next outer iteration (61-65)
+        cpl_snap = np.array(cpl, copy=True) if self.user_callback is not None else None
         device_vtd = isinstance(self.Vtd, BasisVtd)
         if device_vtd:
             dev.vtd_set(self.Vtd.basis, list(self.Vtrap))
         try:
             for j in range(start + 1, niter_outer + 1):
-                if self.coupling is not cpl:                   # callbacks may change it (61-65)
+                if self.coupling is not cpl or (cpl_snap is not None and not np.array_equal(cpl, cpl_snap)):
                     cpl = self.coupling
                     self._upload_coupling()
+                    if self.user_callback is not None:
+                        cpl_snap = np.array(cpl, copy=True)
                 if static:
                     time2, mu, ncur = dev.split_outer(j)
                 else:

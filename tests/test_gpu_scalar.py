@@ -243,3 +243,39 @@ def test_outer_graph_replays_equal_direct_launches(dims, n, xmax, monkeypatch):
     assert len(seen["graph"]) == len(seen["direct"]) == 4
     for (a, ea), (b, eb) in zip(seen["graph"], seen["direct"]):
         assert np.array_equal(a, b) and ea == eb
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_get_energy_halves_g_uniform_state_gpu(dims):
+    """Uniform psi of norm N in V = 0: get_energy = g N^2/(2 Vol) and the chemical-potential
+    integral g N^2/Vol (GPEtask.m:97-107 halves g; the kinetic term of a constant is 0)."""
+    import gpe_hip
+    from closed_forms import uniform_energy
+    if dims == 2:
+        grid = gpe_hip.grid2d(5, 16, 4, 12)
+        vol = 16 * 12 * grid.dx * grid.dy
+    else:
+        grid = gpe_hip.grid3d(5, 16, 4, 12, 3, 8)
+        vol = 16 * 12 * 8 * grid.dx * grid.dy * grid.dz
+    t = gpe_hip.GPEtask(grid, 0 * grid.mesh.x)
+    t.g = 0.7
+    N = 3.0
+    t.Ntotal = N
+    psi = np.sqrt(N / vol) * np.exp(0.3j) * np.ones(grid.size)
+    E, M = uniform_energy(t.g, N, vol)
+    assert abs(t.get_energy(psi) / E - 1) < 1e-12
+    assert abs(t.chemical_potential_integral(psi) / M - 1) < 1e-12
+
+
+def test_get_energy_interaction_identity_gpu():
+    """get_energy - <psi|H psi> = -(g/2) sum |psi|^4 w for a generic state at g != 0."""
+    import gpe_hip
+    grid = gpe_hip.grid2d(6, 32, 5, 24)
+    t = gpe_hip.GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + 2 * Y ** 2))
+    t.g = 0.9
+    t.Ntotal = 1.0
+    rng = np.random.Generator(np.random.PCG64(5))
+    psi = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 3) * np.exp(1j * rng.uniform(0, 1, grid.size))
+    w = grid.dx * grid.dy
+    d = t.get_energy(psi) - t.chemical_potential_integral(psi)
+    assert abs(d / (-0.5 * t.g * np.sum(np.abs(psi) ** 4) * w) - 1) < 1e-11

@@ -437,3 +437,113 @@ def test_sgpe_energy_cutoff_empties_the_masked_modes():
     phi = s.solve_split(0.01, 10, 1)
     spec = np.abs(grid.fft(phi)) ** 2
     assert spec[grid.kk >= 4.0].max() < 1e-25 * spec.max() and spec[grid.kk < 4.0].max() > 0.0
+
+
+# --------------------------------------------------------------- complex Omega and g/2
+# The Rabi step's complex phase convention (GPEtaskSpinor/solve_split.m:61-65, 90-91
+# against the Hamiltonian of GPEtaskSpinor.m:66-70) and the halved interaction of
+# get_energy (GPEtask.m:97-107, GPEtaskSpinor.m:100-111), pinned by closed forms.
+from closed_forms import (grid_wavenumber, rabi_two_mode, two_mode_state, uniform_energy,  # noqa: E402
+                          uniform_spinor_energy)
+
+
+def test_spinor_complex_omega_is_exp_minus_iHT():
+    """V = g = 0, uniform psi1 = 1, psi2 = 0, Omega = |Omega| e^{i th}: the step is
+    exp(-i H T) for H = [[0, Omega], [conj(Omega), 0]] exactly (products of rotations),
+    so psi1 = cos(|Omega| T), psi2 = -i e^{-i th} sin(|Omega| T), and mu stays 0
+    (Re(Omega conj(psi1) psi2) = 0).  Swapping e^{+i th} and e^{-i th} gives psi2 a
+    phase e^{+i th} and fails."""
+    grid = Grid2D(5, 8, 5, 8)
+    t = GPEtaskSpinor(grid, [0 * grid.mesh.x, 0 * grid.mesh.x])
+    t.g = np.zeros((2, 2))
+    r, th = 0.5, 0.7
+    t.coupling = r * np.exp(1j * th)
+    t.Ntotal = grid.integrate(np.ones(grid.size))
+    t.init_state = [np.ones(grid.size, complex), np.zeros(grid.size, complex)]
+    out = t.solve_split(0.01, 30, 4)
+    T = 1.2
+    assert rel_l2(out[0], np.cos(r * T) * np.ones(grid.size)) < 1e-13
+    assert rel_l2(out[1], -1j * np.exp(-1j * th) * np.sin(r * T) * np.ones(grid.size)) < 1e-13
+    assert np.all(np.abs(t.history["mu"]) < 1e-14)
+
+
+@pytest.mark.parametrize("r,m,th,gamma", [(0.5, 2, 0.7, 0.0), (0.5, -3, -2.1, 0.0), (2.0, 1, 0.3, 0.02),
+                                          (500.0, 2, 0.4, 0.0)])
+def test_spinor_space_dependent_omega_two_mode_reduction(r, m, th, gamma):
+    """Omega = r e^{i(qx + th)} with q a grid wave number keeps psi in the two-mode
+    subspace (psi1 = a, psi2 = b e^{-i(qx+th)}); the oracle must equal the 2x2 product
+    of the reduction (tests/closed_forms.py), including mu, n and the damped
+    renormalisation.  r = 500 puts |Omega| dt = 1 (the large-angle regime)."""
+    grid = Grid2D(5, 16, 4, 12)
+    q = grid_wavenumber(16, 5, m)
+    t = GPEtaskSpinor(grid, [0 * grid.mesh.x, 0 * grid.mesh.x])
+    t.g = np.zeros((2, 2))
+    t.gamma = gamma
+    t.coupling = r * np.exp(1j * (q * grid.mesh.x + th))
+    vol = 16 * 12 * grid.dx * grid.dy
+    a0, b0 = 0.8, 0.6j
+    t.Ntotal = (abs(a0) ** 2 + abs(b0) ** 2) * vol
+    t.init_state = list(two_mode_state(grid, q, th, a0, b0))
+    ddt = 0.002
+    out = t.solve_split(ddt, 20, 3)
+    a, b, mus, ns = rabi_two_mode(r, q, ddt, 20, 3, 10, vol, t.Ntotal, gamma, a0, b0)
+    e1, e2 = two_mode_state(grid, q, th, a, b)
+    assert rel_l2(out[0], e1) < 1e-12 and rel_l2(out[1], e2) < 1e-12
+    assert np.allclose(t.history["mu"], mus, rtol=1e-11, atol=1e-11 * abs(mus).max())
+    assert np.allclose(t.history["n"], ns, rtol=1e-12)
+
+
+def test_two_mode_reduction_matches_exp_minus_iHT():
+    """The reduction itself: q = 0, gamma = 0, mu = 0 gives a = cos(rT), b = -i sin(rT)."""
+    a, b, mus, _ = rabi_two_mode(0.5, 0.0, 0.01, 30, 4, 10, 1.0, 1.0)
+    assert abs(a - np.cos(0.6)) < 1e-14 and abs(b + 1j * np.sin(0.6)) < 1e-14
+    assert np.all(np.abs(mus) < 1e-15)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_get_energy_halves_g_uniform_state(dims):
+    """Uniform psi of norm N in V = 0 on the periodic box: get_energy = g N^2/(2 Vol),
+    <psi|H psi> = g N^2/Vol (the kinetic term of a constant is 0)."""
+    if dims == 2:
+        grid = Grid2D(5, 16, 4, 12)
+        vol = 16 * 12 * grid.dx * grid.dy
+    else:
+        grid = Grid3D(5, 16, 4, 12, 3, 8)
+        vol = 16 * 12 * 8 * grid.dx * grid.dy * grid.dz
+    t = GPEtask(grid, 0 * grid.mesh.x)
+    t.g = 0.7
+    N = 3.0
+    psi = np.sqrt(N / vol) * np.exp(0.3j) * np.ones(grid.size)
+    E, M = uniform_energy(t.g, N, vol)
+    assert abs(t.get_energy(psi) / E - 1) < 1e-13
+    assert abs(np.real(grid.inner(psi, t.applyham(psi))) / M - 1) < 1e-13
+
+
+def test_get_energy_interaction_identity():
+    """Any state: get_energy - <psi|H psi> = -(g/2) sum |psi|^4 w at g != 0."""
+    grid = Grid2D(6, 32, 5, 24)
+    t = GPEtask(grid, lambda X, Y, Z: 0.5 * (X ** 2 + 2 * Y ** 2))
+    t.g = 0.9
+    rng = np.random.Generator(np.random.PCG64(5))
+    psi = np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 3) * np.exp(1j * rng.uniform(0, 1, grid.size))
+    w = grid.dx * grid.dy
+    E = t.get_energy(psi)
+    M = np.real(grid.inner(psi, t.applyham(psi)))
+    assert abs((E - M) / (-0.5 * t.g * np.sum(np.abs(psi) ** 4) * w) - 1) < 1e-12
+
+
+def test_spinor_get_energy_uniform_closed_form():
+    """Uniform components a, b in V = 0 with uniform Omega: (1/2) sum_ik g_ik n_i n_k / Vol
+    plus the coupling energy 2 Re(Omega conj(a) b) Vol (not halved)."""
+    grid = Grid2D(5, 16, 4, 12)
+    vol = 16 * 12 * grid.dx * grid.dy
+    t = GPEtaskSpinor(grid, [0 * grid.mesh.x, 0 * grid.mesh.x])
+    gm = np.array([[0.5, 0.45], [0.4, 0.6]])
+    t.g = gm
+    om = 0.3 * np.exp(0.9j)
+    t.coupling = om
+    a, b = 0.7 * np.exp(0.2j), 0.4 * np.exp(-1.1j)
+    phi = [a * np.ones(grid.size), b * np.ones(grid.size)]
+    n1, n2 = abs(a) ** 2 * vol, abs(b) ** 2 * vol
+    E = uniform_spinor_energy(gm, n1, n2, om, a, b, vol)
+    assert abs(t.get_energy(phi) / E - 1) < 1e-13
