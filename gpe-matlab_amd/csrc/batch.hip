@@ -589,6 +589,11 @@ int gpe_batch_split_outer(gpe_batch* b, int j, double* time2_out, double* muc, d
     if (muc) muc[i] = s[i].muc;
     if (ncur) ncur[i] = s[i].ncur;
   }
+  // the block end divides by each trajectory's norm (solve_split.m:67-73)
+  for (int i = 0; i < ntraj; ++i)
+    if (!std::isfinite(s[i].ncur) || !std::isfinite(s[i].muc) || !std::isfinite(s[i].scale) || !std::isfinite(s[i].mu))
+      return bfail(b, GPE_ENONFINITE, "trajectory %d: block reduction is not finite (ncur %g, mu %g)", i, s[i].ncur,
+                   s[i].muc);
   return GPE_OK;
 }
 

@@ -450,6 +450,7 @@ int gpe_split_block(gpe_handle* h, double time2, double* muc, double* ncur) {
   if ((rc = run_block(h, time2))) return rc;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
+  if ((rc = check_finite(h, s, "gpe_split_block"))) return rc;
   if (muc) *muc = s.muc;
   if (ncur) *ncur = s.ncur;
   return GPE_OK;
@@ -478,6 +479,7 @@ int gpe_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, double
   time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
+  if ((rc = check_finite(h, s, "gpe_split_outer"))) return rc;
   if (time2_out) *time2_out = time2;
   if (muc) *muc = s.muc;
   if (ncur) *ncur = s.ncur;
@@ -561,6 +563,9 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
       HIPCHK(h, gpe::launch_finalize(fx, h->stream));
       DevState t;
       if ((rc = read_state(h, &t))) return rc;
+      if (!std::isfinite(t.delta) || !std::isfinite(t.mu_itp))
+        return fail(h, GPE_ENONFINITE, "gpe_itp: iteration %d is not finite (delta %g, norm factor %g)", i, t.delta,
+                    t.mu_itp);
       if (t.delta < a->eps) {                                      // 63
         if (a->nargout >= 3) {
           double mui = 0.0;
@@ -586,6 +591,7 @@ int gpe_itp(gpe_handle* h, const gpe_itp_args* a, double* MU, double* MU2, doubl
   if (dt_final) *dt_final = dt;
   if (converged) *converged = conv;
   h->begun = false;
+  if (!std::isfinite(MU[i - 1])) return fail(h, GPE_ENONFINITE, "gpe_itp: chemical potential is not finite");
   return GPE_OK;
 }
 

@@ -418,6 +418,16 @@ inline int read_state(gpe_handle* h, DevState* out) {
   return GPE_OK;
 }
 
+// The block end divides by the norm (solve_split.m:67-73, phi*sqrt(NNN/ncur)) and the
+// chemical potential feeds the next block's phase: once either is NaN or Inf the state
+// is no longer finite, and the run is stopped with GPE_ENONFINITE instead of continuing.
+inline int check_finite(gpe_handle* h, const DevState& s, const char* where) {
+  if (!std::isfinite(s.ncur) || !std::isfinite(s.muc) || !std::isfinite(s.scale) || !std::isfinite(s.mu))
+    return fail(h, GPE_ENONFINITE, "%s: block reduction is not finite (ncur %g, mu %g, scale %g)", where, s.ncur,
+                s.muc, s.scale);
+  return GPE_OK;
+}
+
 inline int write_state(gpe_handle* h, const DevState& s) {
   HIPCHK(h, hipMemcpyAsync(h->st, &s, sizeof(DevState), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -144,6 +144,8 @@ extern "C" int gpe_itp2(gpe_handle* h, const gpe_itp2_args* a, double* MU, doubl
       HIPCHK(h, hipStreamSynchronize(h->stream));
       const double e_i = ebuf[9], e_i1 = ebuf[8], e_i9 = ebuf[0];
       const double delta = (std::fabs(e_i - e_i9) / 9 + std::fabs(e_i - e_i1)) / dt;
+      if (!std::isfinite(delta))
+        return fail(h, GPE_ENONFINITE, "gpe_itp2: iteration %d is not finite (convergence series %g)", i, e_i);
       if (delta < a->eps) {
         if (dt < a->eps || dt < 5e-6) break;
         dt = dt / 1.5;
@@ -179,5 +181,6 @@ extern "C" int gpe_itp2(gpe_handle* h, const gpe_itp2_args* a, double* MU, doubl
   if (dt_final) *dt_final = dt;
   if (converged) *converged = conv;
   h->begun = false;
+  if (nh && !std::isfinite(MU[i - 2])) return fail(h, GPE_ENONFINITE, "gpe_itp2: chemical potential is not finite");
   return GPE_OK;
 }

@@ -49,6 +49,7 @@ int sg_block(gpe_handle* h, long long step0, double* ncur) {
   if (ncur) {
     DevState s;
     if ((rc = read_state(h, &s))) return rc;
+    if (!std::isfinite(s.sA)) return fail(h, GPE_ENONFINITE, "SGPE block norm is not finite (%g)", s.sA);
     *ncur = s.sA * h->w;
   }
   return GPE_OK;

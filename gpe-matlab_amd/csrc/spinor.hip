@@ -258,6 +258,7 @@ int spinor_split_block(gpe_handle* h, double time2, double* muc, double* ncur) {
   if ((rc = spinor_block(h, time2, nullptr, true))) return rc;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
+  if ((rc = check_finite(h, s, "gpe_split_block"))) return rc;
   if (muc) *muc = s.muc;
   if (ncur) *ncur = s.ncur;
   return GPE_OK;
@@ -285,6 +286,7 @@ int spinor_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, dou
   time2 = time + (double)(nblocks - 1) * h->ddt * (double)h->n_rec;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
+  if ((rc = check_finite(h, s, "gpe_split_outer"))) return rc;
   if (time2_out) *time2_out = time2;
   if (muc) *muc = s.muc;
   if (ncur) *ncur = s.ncur;

@@ -11,10 +11,10 @@
 The solvers run through the C ABI of libgpehip.so (include/gpe_hip.h); the
 package raises GPELibraryError when that library is missing.
 """
-from ._lib import GPELibraryError, device_info, load  # noqa: F401
+from ._lib import GPELibraryError, GPENonFiniteError, device_info, load  # noqa: F401
 from .grid import grid2d, grid3d  # noqa: F401
 from .task import GPEtask, GPEtask2comp, GPEtaskSpinor, SGPEtask  # noqa: F401
 from .diagnostics import detect_core  # noqa: F401
 from .vtd import BasisVtd  # noqa: F401
 
-__all__ = ["grid2d", "grid3d", "GPEtask", "GPEtask2comp", "GPEtaskSpinor", "SGPEtask", "BasisVtd", "detect_core", "GPELibraryError", "device_info", "load"]
+__all__ = ["grid2d", "grid3d", "GPEtask", "GPEtask2comp", "GPEtaskSpinor", "SGPEtask", "BasisVtd", "detect_core", "GPELibraryError", "GPENonFiniteError", "device_info", "load"]

@@ -19,6 +19,18 @@ class GPELibraryError(RuntimeError):
     pass
 
 
+class GPENonFiniteError(GPELibraryError):
+    """GPE_ENONFINITE: a block reduction (norm, chemical potential) came out NaN or Inf,
+    so the state is no longer finite; the solver stops at that outer iteration."""
+
+
+GPE_ENONFINITE = -5
+
+
+def _error(rc, text):
+    return (GPENonFiniteError if rc == GPE_ENONFINITE else GPELibraryError)(text)
+
+
 class GridDesc(C.Structure):
     _fields_ = [("ndims", C.c_int), ("n", C.c_int * 3), ("xmax", C.c_double * 3)]
 
@@ -153,7 +165,7 @@ def check(rc, h=None, what=""):
         if h is not None and _lib is not None:
             m = _lib.gpe_last_error(h)
             msg = m.decode() if m else ""
-        raise GPELibraryError(f"{what}: {ERRNAMES.get(rc, rc)} {msg}")
+        raise _error(rc, f"{what}: {ERRNAMES.get(rc, rc)} {msg}")
 
 
 def to_colmajor_c128(a):
@@ -485,7 +497,7 @@ class Batch:
         rc = getattr(self.lib, name)(self.h, *args)
         if rc != GPE_OK:
             m = self.lib.gpe_batch_last_error(self.h)
-            raise GPELibraryError(f"{name}: {ERRNAMES.get(rc, rc)} {m.decode() if m else ''}")
+            raise _error(rc, f"{name}: {ERRNAMES.get(rc, rc)} {m.decode() if m else ''}")
 
     def set(self, i, params, V, psi):
         v = to_colmajor_f64(np.broadcast_to(V, self.shapes[i]))
