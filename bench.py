@@ -380,19 +380,14 @@ def bench_c1(args):
 def bench_c3(args):
     """Adjoint gradient of 1 - |<psi_t|psi(T)>|^2 w.r.t. 64 control amplitudes, 512^2, T = 2000."""
     import gpe_hip
+    from gpe_hip import workloads
     from gpe_hip.adjoint import SplitStepControl
-    rng = np.random.Generator(np.random.PCG64(2))
-    G = gpe_hip.grid2d(20, 512, 20, 512)
-    V0 = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2)
-    ab = rng.uniform(-8, 8, (8, 2))
-    basis = [np.exp(-((G.mesh.x - a) ** 2 + (G.mesh.y - b) ** 2) / 2) for a, b in ab]
-    c = rng.normal(0, 0.5, (8, 8))
-    ct = np.random.Generator(np.random.PCG64(3)).normal(0, 0.5, (8, 8))
-    psi0 = G.normalize(np.exp(-((G.mesh.x - 1) ** 2 + G.mesh.y ** 2) / 2) + 0j)
+    w = workloads.c3_control()
+    G, V0, basis, c, ct = w["grid"], w["V0"], w["basis"], w["c"], w["c_target"]
     t = gpe_hip.GPEtask(G, V0)
-    t.g, t.Ntotal = 10.0, 1.0
-    t.init_state = psi0
-    T, ddt, spi = 2000, 0.002, 250
+    t.g, t.Ntotal = w["g"], w["Ntotal"]
+    t.init_state = w["psi0"]
+    T, ddt, spi = w["nsteps"], w["ddt"], w["spi"]
     ctl = SplitStepControl(t, V0, basis, ddt, T, spi)
     psi_t = ctl.forward(ct)
     ctl.forward(c)
@@ -510,14 +505,8 @@ def bench_c5(args):
     import gpe_hip
     from gpe_hip import workloads
     n = 192
-    G = gpe_hip.grid3d(12, n, 12, n, 12, n)
-    V = 0.5 * (G.mesh.x ** 2 + G.mesh.y ** 2 + G.mesh.z ** 2)
-    tf, _ = workloads.thomas_fermi(G, V, 1e6, 0.02)
-    rng = np.random.Generator(np.random.PCG64(5))
-    a = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
-    bcomp = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, G.size))
-    gm = np.array([[0.02, 0.018], [0.018, 0.02]])
-    om = 0.5 * np.exp(2j * G.mesh.x)
+    w = workloads.c5_spinor(n=n)
+    G, V, (a, bcomp), gm, om = w["grid"], w["V"], w["psi"], w["g"], w["coupling"]
     t = gpe_hip.GPEtaskSpinor(G, [V, V])
     t.g, t.coupling, t.Ntotal = gm, om, 1e6
     t.init_state = [a, bcomp]

@@ -79,3 +79,37 @@ def imprint_vortex_at(grid, phi, g, x0, y0):
     xi = 1 / np.sqrt(2 * g * np.max(np.abs(phi)) ** 2)
     rr = np.sqrt((grid.mesh.x - x0) ** 2 + (grid.mesh.y - y0) ** 2)
     return phi * np.tanh(rr / xi) * np.exp(1j * np.arctan2(grid.mesh.x - x0, grid.mesh.y - y0))
+
+
+def c3_control(n=512, xmax=20.0, nsteps=2000, spi=250):
+    """Adjoint workload: grid2d(20, 512, 20, 512), V = (X^2+Y^2)/2 + sum_k c[k, m] G_k with
+    eight Gaussian bumps G_k = exp(-((X-a_k)^2+(Y-b_k)^2)/2), (a_k, b_k) ~ U[-8, 8] and
+    c ~ N(0, 0.5^2) over eight intervals of spi steps (seed 2); the target state is the
+    same dynamics under an independent draw of c (seed 3).  g = 10, Ntotal = 1,
+    gamma = 0, dt = 0.002, T = nsteps steps.  psi0 is a unit-norm Gaussian at x = 1."""
+    grid = grid2d(xmax, n, xmax, n)
+    rng = np.random.Generator(np.random.PCG64(2))
+    V0 = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    ab = rng.uniform(-8, 8, (8, 2))
+    basis = [np.exp(-((grid.mesh.x - a) ** 2 + (grid.mesh.y - b) ** 2) / 2) for a, b in ab]
+    nint = (nsteps + spi - 1) // spi
+    c = rng.normal(0, 0.5, (8, nint))
+    ct = np.random.Generator(np.random.PCG64(3)).normal(0, 0.5, (8, nint))
+    psi0 = grid.normalize(np.exp(-((grid.mesh.x - 1) ** 2 + grid.mesh.y ** 2) / 2) + 0j)
+    return dict(grid=grid, grid_args=(xmax, n, xmax, n), V0=V0, basis=basis, c=c, c_target=ct, psi0=psi0,
+                g=10.0, Ntotal=1.0, ddt=0.002, nsteps=nsteps, spi=spi)
+
+
+def c5_spinor(n=192, xmax=12.0, seed=5):
+    """Two-component workload: grid3d(12, 192, ...), V1 = V2 = (X^2+Y^2+Z^2)/2,
+    g = [0.02 0.018; 0.018 0.02], Omega = 0.5 exp(2iX), Ntotal = 1e6; each component is
+    the Thomas-Fermi profile / sqrt(2) times exp(i theta), theta ~ U[0, 0.1] (seeded)."""
+    grid = grid3d(xmax, n, xmax, n, xmax, n)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2 + grid.mesh.z ** 2)
+    tf, _ = thomas_fermi(grid, V, 1e6, 0.02)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, grid.size))
+    b = tf * np.sqrt(0.5) * np.exp(1j * rng.uniform(0, 0.1, grid.size))
+    gm = np.array([[0.02, 0.018], [0.018, 0.02]])
+    return dict(grid=grid, grid_args=(xmax, n, xmax, n, xmax, n), V=V, psi=[a, b], g=gm,
+                coupling=0.5 * np.exp(2j * grid.mesh.x), omega0=0.5, k=2.0, Ntotal=1e6)
