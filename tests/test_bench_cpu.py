@@ -16,12 +16,12 @@ def _bench():
 
 
 @pytest.mark.parametrize("steps,warmup,expect", [
-    ((20, 5), None, (10, 1, 2)),      # the round-end command: 20 timed steps, one warm-up block
-    ((7, 3), None, (10, 1, 1)),
-    ((500, 50), None, (50, 1, 10)),
-    ((1, 0), None, (10, 1, 1)),       # at least one warm-up block (graph capture)
-    ((100, 50), None, (50, 1, 2)),
-    ((30, 20), None, (10, 2, 3)),
+    ((20, 5), None, (10, 10, 2)),     # 20 timed steps after the minimum warm-up of 100
+    ((7, 3), None, (10, 10, 1)),
+    ((500, 50), None, (50, 2, 10)),
+    ((1, 0), None, (10, 10, 1)),
+    ((100, 50), None, (50, 2, 2)),
+    ((30, 200), None, (10, 20, 3)),
 ])
 def test_plan_steps(steps, warmup, expect):
     b = _bench()
@@ -29,7 +29,7 @@ def test_plan_steps(steps, warmup, expect):
     assert (ni, wo, to) == expect
     assert ni % 10 == 0                       # whole n_recalc blocks: solve_split keeps n_rec = 10
     assert ni * to >= steps[0] and ni * to - steps[0] < 10
-    assert ni * wo >= steps[1]
+    assert ni * wo >= steps[1] and ni * wo >= b.MIN_WARMUP
 
 
 def test_plan_steps_never_exits():
