@@ -18,6 +18,7 @@
 // detect_core.m:56-59 in closed form: det(A - d B) = 0 is a quadratic in d and v is
 // the null vector of A - d B, taken from its row of larger magnitude.
 #include <algorithm>
+#include <mutex>
 
 #include "handle.h"
 
@@ -322,11 +323,47 @@ __global__ void __launch_bounds__(DC_T) dc_emit_kernel(DcSlice s, DcCoords xy, c
 
 namespace {
 
+// Scratch of gpe_detect_core_noalloc: the slice's intermediate fields (angle, del2,
+// plaquette flags, group maxima, counts, offsets) and the compacted output, in a
+// fixed device array instead of the handle's grown allocation.  Each GPU the library
+// runs on holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: dc_scratch() looks up the device address once per
+// device with hipGetSymbolAddress and every launch and copy takes that pointer.
+// No part needs clearing between calls: every launch writes each value the next one
+// reads.  Calls that overlap on one GPU (two threads, or handles on other streams)
+// share the array and race.
+constexpr size_t DC_SCRATCH_BYTES = (size_t)GPE_DC_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) char scratch[DC_SCRATCH_BYTES];
+
+constexpr int DC_MAX_DEVICES = 64;
+std::mutex dc_scratch_mu;
+char* dc_scratch_ptr[DC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device h->device (looked up on the
+// first call on that device).
+int dc_scratch(gpe_handle* h, char** out) {
+  if (h->device < 0 || h->device >= DC_MAX_DEVICES) return fail(h, GPE_EINVAL, "device %d out of range", h->device);
+  std::lock_guard<std::mutex> lock(dc_scratch_mu);
+  if (!dc_scratch_ptr[h->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(h, GPE_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    dc_scratch_ptr[h->device] = static_cast<char*>(p);
+  }
+  *out = dc_scratch_ptr[h->device];
+  return GPE_OK;
+}
+
 size_t dc_align(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Runs the five launches on the slice `psi` (device) and copies the results out.
+// fixed == nullptr: the handle's scratch (dc_buf), grown with hipMalloc when the
+// slice needs more; otherwise the fixed_bytes at `fixed`, and a slice that needs more
+// is refused (nothing is allocated on this path, on the device or the host).
 int dc_run(gpe_handle* h, const double2* psi, bool upload, const double* phi_host, double scale, int ny, int nx,
-           const double* x, const double* y, int refine, int cap, double* resp, int* np, double* resm, int* nm) {
+           const double* x, const double* y, int refine, int cap, double* resp, int* np, double* resm, int* nm,
+           char* fixed = nullptr, size_t fixed_bytes = 0) {
   if (!x || !y || !np || !nm || (cap > 0 && (!resp || !resm)) || cap < 0)
     return fail(h, GPE_EINVAL, "null coordinates / outputs or negative capacity");
   if (ny < 4 || nx < 4) return fail(h, GPE_EINVAL, "detect_core needs at least 4 points per axis");
@@ -356,14 +393,23 @@ int dc_run(gpe_handle* h, const double2* psi, bool upload, const double* phi_hos
   const size_t out_bytes = 16 + (size_t)4 * cap * sizeof(double);
   off += dc_align(out_bytes);
   HIPCHK(h, hipSetDevice(h->device));
-  if (off > h->dc_bytes) {
-    (void)hipFree(h->dc_buf);
-    h->dc_buf = nullptr;
-    h->dc_bytes = 0;
-    HIPCHK(h, hipMalloc(&h->dc_buf, off));
-    h->dc_bytes = off;
+  char* B = fixed;
+  if (fixed) {
+    if (off > fixed_bytes)
+      return fail(h, GPE_EINVAL,
+                  "gpe_detect_core_noalloc: a %d x %d slice with capacity %d needs %zu bytes of scratch, more than "
+                  "the %zu it holds",
+                  ny, nx, cap, off, fixed_bytes);
+  } else {
+    if (off > h->dc_bytes) {
+      (void)hipFree(h->dc_buf);
+      h->dc_buf = nullptr;
+      h->dc_bytes = 0;
+      HIPCHK(h, hipMalloc(&h->dc_buf, off));
+      h->dc_bytes = off;
+    }
+    B = h->dc_buf;
   }
-  char* B = h->dc_buf;
   if (upload) {
     HIPCHK(h, hipMemcpyAsync(B + o_phi, phi_host, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
     psi = reinterpret_cast<const double2*>(B + o_phi);
@@ -396,6 +442,18 @@ int dc_run(gpe_handle* h, const double2* psi, bool upload, const double* phi_hos
   hipLaunchKernelGGL(gpe::dc_emit_kernel, dim3(nb), dim3(gpe::DC_T), 0, h->stream, s, c, q, offs, refine, cap, outp,
                      outm);
   HIPCHK(h, hipGetLastError());
+  if (fixed) {   // counts first, then exactly the rows that fit, straight into the caller's arrays
+    int t[4] = {0, 0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(t, B + o_out, sizeof t, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *np = t[0];
+    *nm = t[1];
+    const size_t kp = (size_t)std::min(t[0], cap), km = (size_t)std::min(t[1], cap);
+    if (kp) HIPCHK(h, hipMemcpyAsync(resp, outp, 2 * kp * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (km) HIPCHK(h, hipMemcpyAsync(resm, outm, 2 * km * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GPE_OK;
+  }
   std::vector<char> host(out_bytes);
   HIPCHK(h, hipMemcpyAsync(host.data(), B + o_out, out_bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -427,6 +485,23 @@ int gpe_detect_core(gpe_handle* h, int comp, int zslice, const double* x, const 
   if ((rc = read_state(h, &st))) return rc;
   const double2* psi = h->psi[comp] + (size_t)zslice * h->nx * h->ny;
   return dc_run(h, psi, false, nullptr, st.scale, h->ny, h->nx, x, y, refine, cap, resp, np, resm, nm);
+}
+
+int gpe_detect_core_noalloc(gpe_handle* h, int comp, int zslice, const double* x, const double* y, int refine,
+                            int cap, double* resp, int* np, double* resm, int* nm) {
+  if (!h) return GPE_EINVAL;
+  if (comp < 0 || comp > 1 || !h->psi[comp]) return fail(h, GPE_EINVAL, "bad component");
+  const int nz = h->ndims == 3 ? h->nz : 1;
+  if (zslice < 0 || zslice >= nz) return fail(h, GPE_EINVAL, "z slice %d outside 0..%d", zslice, nz - 1);
+  HIPCHK(h, hipSetDevice(h->device));
+  char* base = nullptr;
+  int rc;
+  if ((rc = dc_scratch(h, &base))) return rc;
+  DevState st;
+  if ((rc = read_state(h, &st))) return rc;
+  const double2* psi = h->psi[comp] + (size_t)zslice * h->nx * h->ny;
+  return dc_run(h, psi, false, nullptr, st.scale, h->ny, h->nx, x, y, refine, cap, resp, np, resm, nm, base,
+                DC_SCRATCH_BYTES);
 }
 
 int gpe_detect_core_array(gpe_handle* h, const double* phi, int ny, int nx, const double* x, const double* y,

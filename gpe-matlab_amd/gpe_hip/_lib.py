@@ -89,6 +89,7 @@ SIGNATURES = {
     "gpe_sgpe_outer": [_P, C.c_int, _DP, _DP],
     "gpe_detect_core": [_P, C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_int, _DP, _IP, _DP, _IP],
     "gpe_detect_core_array": [_P, _DP, C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_int, _DP, _IP, _DP, _IP],
+    "gpe_detect_core_noalloc": [_P, C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_int, _DP, _IP, _DP, _IP],
     "gpe_vtd_set": [_P, C.c_int, _DP, _DP],
     "gpe_vtd_coeffs": [_P, _DP, C.c_int],
     "gpe_set_potential_mu": [_P, C.c_int, _DP],

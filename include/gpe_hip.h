@@ -213,6 +213,18 @@ int gpe_detect_core(gpe_handle* h, int comp, int zslice, const double* x, const 
 int gpe_detect_core_array(gpe_handle* h, const double* phi, int ny, int nx, const double* x, const double* y,
                           int refine, int cap, double* resp, int* np, double* resm, int* nm);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc): gpe_detect_core with the same
+ * arguments and results, whose intermediate fields live in a fixed device array of
+ * GPE_DC_NOALLOC_SCRATCH_MIB MiB per GPU (held until the process ends) instead of the
+ * handle's grown buffer; it calls no hipMalloc/hipFree.  A slice whose fields and
+ * output rows need more than that (about 17 bytes per point + 32 * cap bytes; a
+ * 512 x 512 slice fits with cap up to 300 000) returns GPE_EINVAL.  Calls that
+ * overlap on one GPU share the array and race. */
+#define GPE_DC_NOALLOC_SCRATCH_MIB 16
+int gpe_detect_core_noalloc(gpe_handle* h, int comp, int zslice, const double* x, const double* y, int refine,
+                            int cap, double* resp, int* np, double* resm, int* nm);
+
 /* ------------------------------------------------------------------ adjoint
  * (no reference counterpart: the differentiation form of the split step, a gradient
  * entry point that plays the torch.autograd.Function.backward role).
