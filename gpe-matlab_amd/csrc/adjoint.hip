@@ -288,9 +288,8 @@ int adj_x(gpe_handle* h, int seq) {
     e1 = prof_event(h);
     if (e0) hipEventRecord(e0, h->stream);
   }
-  const char* av = getenv("GPE_ADJ_VAR");   // -1 (unset): follow the x-axis plan of the forward sweeps
-  const int var = av ? atoi(av) : h->var[0];
-  hipError_t e = gpe::launch_adj_x(a.n, seq, var, p, h->stream, nullptr);
+  // the x-axis plan of the forward sweeps (8 elements per thread for the 8.8.8 plan of 512)
+  hipError_t e = gpe::launch_adj_x(a.n, seq, h->var[0], p, h->stream, nullptr);
   if (h->prof && e0 && e1) {
     hipEventRecord(e1, h->stream);
     h->prof_recs.push_back({(seq == gpe::A_MID ? gpe::S_MID : seq == gpe::A_OPEN ? gpe::S_OPEN : gpe::S_CLOSE) * 3,

@@ -370,9 +370,8 @@ int gpe_batch_create(int ntraj, const gpe_grid_desc* grids, gpe_batch** out) {
     total_pts += bins.back().first;
   }
   // a shape holding well over a quarter of the points is cut into chains of about a
-  // quarter each, so no single chain is the critical path (GPE_BATCH_SPLIT=0: off)
-  const char* bsp = getenv("GPE_BATCH_SPLIT");
-  if (!(bsp && atoi(bsp) == 0)) {
+  // quarter each, so no single chain is the critical path
+  {
     std::vector<std::pair<double, std::vector<int>>> cut;
     for (auto& bn : bins) {
       const int cnt = (int)bn.second.size();

@@ -43,16 +43,8 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     delete h;
     return GPE_EHIP;
   }
-  const char* snake = getenv("GPE_SNAKE");
-  h->snake = snake ? atoi(snake) : 0;
-  const char* slabs = getenv("GPE_SLABS");
-  h->slabs = slabs ? atoi(slabs) : 0;
   const char* ys = getenv("GPE_YSPLIT");
   h->ysplit_off = !(ys && atoi(ys) == 1);   // the two-sweep step is opt-in (GPE_YSPLIT=1)
-  const char* yna = getenv("GPE_YSPLIT_NA");
-  if (yna && (atoi(yna) == 1 || atoi(yna) == 2)) h->ysplit_na = atoi(yna);
-  const char* ypv = getenv("GPE_YSPLIT_PLAN");
-  if (ypv && (atoi(ypv) == 0 || atoi(ypv) == 1)) h->ysplit_pv = atoi(ypv);
   h->ndims = d->ndims;
   h->nx = d->n[0];
   h->ny = d->n[1];

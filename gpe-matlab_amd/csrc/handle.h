@@ -125,14 +125,10 @@ struct gpe_handle {
   // y-split 3D step (ny = 256): potential in the x sweeps' tile order, kinetic factor
   // per z pencil in the split's spectral order; v_gen counts writes of V[0]
   bool ysplit_off = true;       // on with GPE_YSPLIT=1
-  int ysplit_na = 1;            // values of a per x-sweep tile (GPE_YSPLIT_NA)
-  int ysplit_pv = 1;            // plan of the x sweeps (GPE_YSPLIT_PLAN)
   double* Vp = nullptr;
   double2* kcolB = nullptr;
   double2* twy = nullptr;       // 256-point twiddles of the y axis
   long long v_gen = 0, vp_gen = -1;
-  int snake = 0;  // bit a: reverse the tile order of the y sweeps (GPE_SNAKE)
-  int slabs = 0;  // > 1: run the plane-local sweeps slab by slab (GPE_SLABS)
   // one outer iteration of gpe_split_outer captured as a graph (static potential,
   // decay_rate = 0); dropped by gpe_split_begin
   hipGraph_t og = nullptr;
@@ -386,7 +382,7 @@ inline int build_kinetic_rot(gpe_handle* h, std::complex<double> dt) {
 // The y-split 3D step can run on this handle (its tables are set up by ysplit_prepare).
 inline bool ysplit_eligible(const gpe_handle* h) {
   return !h->ysplit_off && h->ndims == 3 && h->ny == 256 && h->omega == 0.0 && h->prm.ncomp == 1 &&
-         h->slabs <= 1 && gpe::ysplit_x_supported(h->nx) && gpe::ysplit_x_supported(h->nz) && h->nz % 8 == 0;
+         gpe::ysplit_x_supported(h->nx) && gpe::ysplit_x_supported(h->nz) && h->nz % 8 == 0;
 }
 
 inline int build_step_tables(gpe_handle* h, std::complex<double> dt) {
@@ -406,7 +402,7 @@ inline int ysplit_prepare(gpe_handle* h) {
   if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
   if (h->vp_gen != h->v_gen) {
     HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           (unsigned)h->ysplit_na, h->stream));
+                                           1u, h->stream));
     h->vp_gen = h->v_gen;
   }
   return GPE_OK;
@@ -460,47 +456,6 @@ inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
   return GPE_OK;
 }
 
-// One block of solve_split.m:53-78 (GPEtask, static or pre-uploaded V).
-// Sweep parameters restricted to the z planes [z0, z0 + nzs) of a 3D grid (x or y axis).
-inline SweepParams slab_params(gpe_handle* h, int axis, double2* psi, const double* V, int z0, int nzs) {
-  SweepParams p = base_params(h, axis, psi, V);
-  const size_t off = (size_t)z0 * (size_t)h->nx * (size_t)h->ny;
-  p.psi = psi + off;
-  p.V = V ? V + off : nullptr;
-  p.ncols = (unsigned)((axis == 0 ? h->ny : h->nx) * nzs);
-  return p;
-}
-
-// Block schedule with the plane-local sweeps of one step (y inverse, x phase, y
-// forward) run slab by slab, so a slab stays cache-resident across them; the
-// z kinetic sweep covers the whole grid.  Same arithmetic as run_block.
-inline int run_block_slabs(gpe_handle* h, double time2) {
-  int rc;
-  double2* psi = h->psi[0];
-  const double* V = h->V[0];
-  const int S = h->slabs, nzs = h->nz / h->slabs;
-  for (int i = 0; i < h->n_rec; ++i) {
-    for (int s = 0; s < S; ++s) {
-      if (i > 0 && (rc = sweep(h, 1, gpe::S_INV, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
-      if ((rc = sweep(h, 0, i == 0 ? gpe::S_OPEN : gpe::S_MID, slab_params(h, 0, psi, V, s * nzs, nzs)))) return rc;
-      if ((rc = sweep(h, 1, gpe::S_FWD, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
-    }
-    if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
-  }
-  for (int s = 0; s < S; ++s) {
-    if ((rc = sweep(h, 1, gpe::S_INV, slab_params(h, 1, psi, V, s * nzs, nzs)))) return rc;
-    if ((rc = sweep(h, 0, gpe::S_CLOSE, slab_params(h, 0, psi, V, s * nzs, nzs)))) return rc;
-  }
-  FinParams f = fin_params(h, gpe::FIN_BLOCK);
-  if ((rc = launch_measure(h, 0, &f))) return rc;
-  f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
-  f.NN0 = h->NN0;
-  f.tau = h->prm.decay_rate;
-  f.time2 = time2;
-  HIPCHK(h, gpe::launch_finalize(f, h->stream));
-  return GPE_OK;
-}
-
 // The y-split 3D step applies: one component, no rotation, ny = 256, x length with a
 // y-split x sweep, z length with a 16-pencil-multiple tile, nz a multiple of 8.
 inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy; }
@@ -522,7 +477,7 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
   int rc;
   if (h->vp_gen != h->v_gen) {
     HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           (unsigned)h->ysplit_na, h->stream));
+                                           1u, h->stream));
     h->vp_gen = h->v_gen;
   }
   SweepParams px = ysplit_params(h);
@@ -536,7 +491,7 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
       e1 = prof_event(h);
       if (e0) hipEventRecord(e0, h->stream);
     }
-    hipError_t e = gpe::launch_ysplit_x(h->nx, seq, h->ysplit_na, h->ysplit_pv, px, h->stream, nb);
+    hipError_t e = gpe::launch_ysplit_x(h->nx, seq, px, h->stream, nb);
     if (h->prof && e0 && e1) {
       hipEventRecord(e1, h->stream);
       h->prof_recs.push_back({seq * 3 + 0, {e0, e1}});
@@ -567,7 +522,6 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
 inline int run_block(gpe_handle* h, double time2) {
   const bool rot = h->omega != 0.0;
   if (ysplit_active(h)) return run_block_ysplit(h, time2);
-  if (!rot && h->ndims == 3 && h->slabs > 1 && h->nz % h->slabs == 0) return run_block_slabs(h, time2);
   int rc;
   double2* psi = h->psi[0];
   const double* V = h->V[0];
@@ -576,19 +530,11 @@ inline int run_block(gpe_handle* h, double time2) {
   const int s_open = rot ? gpe::S_ROPEN : gpe::S_OPEN, s_mid = rot ? gpe::S_RMID : gpe::S_MID;
   const int s_close = rot ? gpe::S_RCLOSEM : gpe::S_CLOSEM, s_kin2d = rot ? gpe::S_RKIN : gpe::S_KIN;
   for (int i = 0; i < h->n_rec; ++i) {
-    SweepParams px = base_params(h, 0, psi, V);
-    px.reverse = (h->snake & 4) ? 1 : 0;
-    if ((rc = sweep(h, 0, i > 0 ? s_mid : s_open, px))) return rc;
+    if ((rc = sweep(h, 0, i > 0 ? s_mid : s_open, base_params(h, 0, psi, V)))) return rc;
     if (h->ndims == 3) {
-      SweepParams py = base_params(h, 1, psi, V);
-      py.reverse = (h->snake & 1) ? 1 : 0;
-      if ((rc = sweep(h, 1, gpe::S_FWD, py))) return rc;
-      SweepParams pz = base_params(h, 2, psi, V);
-      pz.reverse = (h->snake & 8) ? 1 : 0;
-      if ((rc = sweep(h, 2, gpe::S_KIN, pz))) return rc;
-      SweepParams pi = base_params(h, 1, psi, V);
-      pi.reverse = (h->snake & 2) ? 1 : 0;
-      if ((rc = sweep(h, 1, gpe::S_INV, pi))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_FWD, base_params(h, 1, psi, V)))) return rc;
+      if ((rc = sweep(h, 2, gpe::S_KIN, base_params(h, 2, psi, V)))) return rc;
+      if ((rc = sweep(h, 1, gpe::S_INV, base_params(h, 1, psi, V)))) return rc;
     } else {
       if ((rc = sweep(h, 1, s_kin2d, base_params(h, 1, psi, V)))) return rc;
     }

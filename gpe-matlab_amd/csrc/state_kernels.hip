@@ -197,17 +197,17 @@ hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_
   }
 }
 
-#define GPE_DECL(N) hipError_t launch_ysplit_x_##N(int, int, int, const SweepParams&, hipStream_t, int*);
+#define GPE_DECL(N) hipError_t launch_ysplit_x_##N(int, const SweepParams&, hipStream_t, int*);
 GPE_DECL(64) GPE_DECL(128) GPE_DECL(256)
 #undef GPE_DECL
 
 bool ysplit_x_supported(int n) { return n == 64 || n == 128 || n == 256; }
 
-hipError_t launch_ysplit_x(int n, int seq, int na, int pv, const SweepParams& p, hipStream_t s, int* nblocks) {
+hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
   switch (n) {
-    case 64: return launch_ysplit_x_64(seq, na, pv, p, s, nblocks);
-    case 128: return launch_ysplit_x_128(seq, na, pv, p, s, nblocks);
-    case 256: return launch_ysplit_x_256(seq, na, pv, p, s, nblocks);
+    case 64: return launch_ysplit_x_64(seq, p, s, nblocks);
+    case 128: return launch_ysplit_x_128(seq, p, s, nblocks);
+    case 256: return launch_ysplit_x_256(seq, p, s, nblocks);
     default: return hipErrorInvalidValue;
   }
 }

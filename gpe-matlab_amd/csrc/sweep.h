@@ -97,7 +97,6 @@ struct SweepParams {
   double g;                  // interaction constant
   double dr, di;             // complex time step dt = dr + i di (ITP: dr = dt, di = 0)
   int apply_scale;           // S_FWD: multiply by the pending scale st->scale first
-  int reverse;               // visit pencil tiles in reverse order (cache reuse between sweeps)
   // rotating frame
   const double2* rtab;       // S_R*: kinetic table [row][i], row = col % trow (incl. 1/N of the axis)
   unsigned trow;             // row modulus of rtab and rcoord
@@ -159,7 +158,7 @@ hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_
 // x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM) for x length n; p.V is the
 // potential in tile order (launch_ysplit_permute_v)
 // na (1 or 2) values of a per tile; pv 0: the phase sweeps' plan, 1: Plan<n>.
-hipError_t launch_ysplit_x(int n, int seq, int na, int pv, const SweepParams& p, hipStream_t s, int* nblocks);
+hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, int* nblocks);
 bool ysplit_x_supported(int n);
 // Vp[(z*(16/na) + ap)*16 na*nx + x*16 na + c] = V[(na ap + c/16 + 16 (c%16)) + 256 x + 256 nx z]
 hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na,

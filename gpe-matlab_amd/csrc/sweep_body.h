@@ -330,20 +330,8 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   }
 }
 
-// Minimum waves per SIMD requested from the register allocator for the kinetic
-// and the phase sweeps (1 = no constraint).
-#ifndef GPE_WAVES_KIN
-#define GPE_WAVES_KIN 1
-#endif
-#ifndef GPE_WAVES_PHASE
-#define GPE_WAVES_PHASE 1
-#endif
-
 template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
-    SEQ == S_KIN ? GPE_WAVES_KIN
-                 : (SEQ == S_MID || SEQ == S_OPEN || SEQ == S_CLOSE || SEQ == S_CLOSEM) ? GPE_WAVES_PHASE : 1)))
-    sweep_kernel(const SweepParams p) {
+__global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
   __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN ? 2 * N : N];
   __shared__ double red[4];
@@ -356,8 +344,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
       psi = p.psi2;
     }
   }
-  const unsigned b = C < 16 ? xcd_pair_tile(bid, nbk) : bid;
-  const unsigned tile = p.reverse ? nbk - 1u - b : b;
+  const unsigned tile = C < 16 ? xcd_pair_tile(bid, nbk) : bid;
   sweep_body<N, C, COLFAST, SEQ, VAR>(p, psi, tile, tile, lds, twl, red);
 }
 

@@ -175,14 +175,12 @@ static hipError_t launch_na(int seq, const SweepParams& p, hipStream_t s, int* n
   return hipGetLastError();
 }
 
-// na: values of a per tile (1 or 2); pv: plan (0 or 1).
-hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, int na, int pv, const SweepParams& p, hipStream_t s,
-                                            int* nblocks) {
+// One value of a per tile (16 pencils) with Plan<N> (16 elements per thread at 256):
+// the fastest of the measured tile/plan combinations.
+hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
   constexpr int N = GPE_N;
   if (p.gny != 256u || p.gnx != (unsigned)N || p.gnz % 8u != 0u || p.gnz == 0u) return hipErrorInvalidValue;
-  if (na == 1) return pv ? launch_na<N, 1, 1>(seq, p, s, nblocks) : launch_na<N, 1, 0>(seq, p, s, nblocks);
-  if (na == 2) return pv ? launch_na<N, 2, 1>(seq, p, s, nblocks) : launch_na<N, 2, 0>(seq, p, s, nblocks);
-  return hipErrorInvalidValue;
+  return launch_na<N, 1, 1>(seq, p, s, nblocks);
 }
 
 }  // namespace gpe

@@ -2,9 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product path (the HIP library, its C-ABI
 and the ``gpe_hip`` host package) imports, links or calls anything here.  Only
-``tests/``, ``__graft_entry__.smoke()``, ``bench.py``'s ``cpu_baseline`` leg and the
-host controls in ``oracle/long_run_controls.py`` use it, and only as the checker / the
-timed CPU baseline / the reference for rounding-drift controls.
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s parity and ``cpu_baseline``
+legs use it, and only as the checker / the timed CPU baseline.
 
 The oracle is a numpy/scipy restatement of the reference MATLAB classes (the
 reference is MATLAB and cannot run here).  Each function cites the reference

@@ -94,24 +94,6 @@ CASES = [((64, 256, 64), (6, 9, 5)), ((128, 256, 64), (7, 10, 5)), ((64, 256, 12
          ((256, 256, 64), (9, 9, 5))]
 
 
-@pytest.mark.parametrize("na,plan", [(1, 1), (1, 0), (2, 1), (2, 0)])
-def test_ysplit_tile_and_plan_options_match_oracle(na, plan):
-    import gpe_hip
-    n, xmax = (128, 256, 64), (7, 10, 5)
-    G, O = _pair(n, xmax)
-    phi = _phi(G, 21)
-    os.environ["GPE_YSPLIT_NA"], os.environ["GPE_YSPLIT_PLAN"] = str(na), str(plan)
-    try:
-        tg = _make(G, gpe_hip, phi, gamma=0.01)
-        tg._device()
-    finally:
-        del os.environ["GPE_YSPLIT_NA"], os.environ["GPE_YSPLIT_PLAN"]
-    to = _make(O, oracle, phi, gamma=0.01)
-    err = rel_l2(tg.solve_split(0.002, 10, 2), to.solve_split(0.002, 10, 2))
-    print(f"y-split na={na} plan={plan}: rel-L2 vs oracle {err:.3e}", flush=True)
-    assert err < TOL
-
-
 @pytest.mark.parametrize("n,xmax", CASES)
 @pytest.mark.parametrize("gamma", [0.0, 0.02])
 def test_ysplit_solve_split_matches_oracle(n, xmax, gamma):
