@@ -478,28 +478,36 @@ def bench_c4(args):
 
 
 def _c4_per_trajectory(ens, ddt, ni, nout, traj_rate, b, log):
-    """The same steps with one handle per trajectory (launch-bound comparison)."""
+    """The same steps with one handle per trajectory, run one trajectory after the
+    other as a loop over tasks does: each handle is created, warmed up (its outer-
+    iteration graph captured), timed over the same outer iterations, checked on a
+    sample against the batched state and destroyed before the next one, so one
+    handle, one stream and one graph exist at a time.  Only the timed outer
+    iterations count."""
+    import gc
     from gpe_hip import _lib
     ens_steps = nout * ni
-    devs = []
-    for gr, V, psi, pr in ens:
+    el1 = 0.0
+    same = True
+    for i, (gr, V, psi, pr) in enumerate(ens):
         d = _lib.Device(gr)
         d.set_params(1, pr["g"], 0.0, 0.0, pr["Ntotal"], 0.0, 10)
         d.set_potential(0, V)
         d.set_state(0, psi)
         d.split_begin(ddt, ni)
-        devs.append(d)
-    for d in devs:
         d.split_outer(1)
-    sync()
-    t0 = time.perf_counter()
-    for d in devs:
+        sync()
+        t0 = time.perf_counter()
         for j in range(2, 2 + nout):
             d.split_outer(j)
-    sync()
-    el1 = time.perf_counter() - t0
+        sync()
+        el1 += time.perf_counter() - t0
+        if i % 37 == 0:
+            same = same and bool(np.array_equal(b.get_state(i), d.get_state(0)))
+        d.close()
+        del d
+        gc.collect()
     single_rate = len(ens) * ens_steps / el1
-    same = all(np.array_equal(b.get_state(i), devs[i].get_state(0)) for i in range(0, len(ens), 37))
     log(f"c4: {len(ens)} trajectories; batched {traj_rate:.0f} traj-steps/s, per-trajectory "
         f"{single_rate:.0f} traj-steps/s, ratio {traj_rate / single_rate:.2f}; bit-identical sample {same}")
     return {"per_trajectory_launch_rate": round(single_rate, 1),
