@@ -38,17 +38,28 @@ def flags():
 
 DEFINES = []
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer for the CPU tests of the C ABI
+# (the device code is built as usual; each sanitizer flag applies to the host pass only).
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer",
+                  "-Xarch_host", "-fsanitize=address",
+                  "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=undefined"]
+SANITIZE = False
+
 
 def compile_one(src, obj, extra=()):
-    cmd = [hipcc(), *flags(), *DEFINES, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *flags(), *DEFINES, *(SANITIZE_FLAGS if SANITIZE else ()), *extra,
+           "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(jobs=None, verbose=True, out=None, build_dir=None):
-    global BUILD, OUT
+def build(jobs=None, verbose=True, out=None, build_dir=None, sanitize=None):
+    global BUILD, OUT, SANITIZE
+    if sanitize is not None:
+        SANITIZE = bool(sanitize)
     if out:
         OUT = out
     if build_dir:
@@ -72,7 +83,8 @@ def build(jobs=None, verbose=True, out=None, build_dir=None):
         for f in futs:
             objs.append(f.result())
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libasan"] if SANITIZE else []
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *san, "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -88,8 +100,14 @@ if __name__ == "__main__":
     ap.add_argument("-D", "--define", action="append", default=[], help="extra -D for every source")
     ap.add_argument("--out", default=None, help="output library path (default gpe_hip/libgpehip.so)")
     ap.add_argument("--build-dir", default=None)
+    ap.add_argument("--sanitize", action="store_true",
+                    help="host-side ASan + UBSan build (into _build/asan/libgpehip_asan.so unless --out)")
     a = ap.parse_args()
     DEFINES.extend(f"-D{d}" for d in a.define)
+    if a.sanitize:
+        SANITIZE = True
+        a.out = a.out or os.path.join(HERE, "_build", "asan", "libgpehip_asan.so")
+        a.build_dir = a.build_dir or os.path.join(HERE, "_build", "asan")
     try:
         build(a.jobs, out=a.out, build_dir=a.build_dir)
     except RuntimeError as e:
