@@ -465,7 +465,15 @@ def bench_c4(args):
     traj_rate = len(ens) * ens_steps / el
     gbps = 72 * npts * (ens_steps / el) / 1e9
     extra = {}
-    if args.no_per_trajectory:
+    profiled = any(k == "ROCP_TOOL_LIBRARIES" or k.startswith("ROCPROF_") for k in os.environ)
+    if profiled and not args.no_per_trajectory:
+        # Under rocprofv3's kernel trace this leg has aborted twice with a malformed AQL
+        # packet during the single-handle outer iterations (once with all 256 handles
+        # alive, once with one handle at a time); it has never failed unprofiled.  It is
+        # not run under the profiler.
+        log("c4: per-trajectory comparison leg skipped under rocprofv3")
+        extra = {"per_trajectory_skipped": "profiler"}
+    if args.no_per_trajectory or profiled:
         log(f"c4: {len(ens)} trajectories, {npts} points; batched {traj_rate:.0f} traj-steps/s")
     else:
         extra = _c4_per_trajectory(ens, ddt, ni, nout, traj_rate, b, log)
