@@ -44,7 +44,7 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     return GPE_EHIP;
   }
   const char* ys = getenv("GPE_YSPLIT");
-  h->ysplit_off = !(ys && atoi(ys) == 1);   // the two-sweep step is opt-in (GPE_YSPLIT=1)
+  h->ysplit_off = ys && atoi(ys) == 0;   // the two-sweep step runs where it applies (GPE_YSPLIT=0: off)
   h->ndims = d->ndims;
   h->nx = d->n[0];
   h->ny = d->n[1];
@@ -616,21 +616,19 @@ int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n) {
 
 int gpe_selftest_rowdft(const double* in, double* out_f, double* out_i) {
   if (!in || !out_f || !out_i) return GPE_EINVAL;
-  double2 *din = nullptr, *df = nullptr, *di = nullptr, *dtw = nullptr;
-  std::vector<double2> tw = twiddles(256);
+  constexpr size_t n = 4096;
+  double2 *din = nullptr, *df = nullptr, *di = nullptr;
   int rc = GPE_EHIP;
-  if (hipMalloc(&din, 128 * sizeof(double2)) == hipSuccess && hipMalloc(&df, 128 * sizeof(double2)) == hipSuccess &&
-      hipMalloc(&di, 128 * sizeof(double2)) == hipSuccess && hipMalloc(&dtw, 256 * sizeof(double2)) == hipSuccess &&
-      hipMemcpy(din, in, 128 * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
-      hipMemcpy(dtw, tw.data(), 256 * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
-      gpe::launch_rowdft_selftest(din, df, di, dtw, nullptr) == hipSuccess &&
-      hipMemcpy(out_f, df, 128 * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess &&
-      hipMemcpy(out_i, di, 128 * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess)
+  if (hipMalloc(&din, n * sizeof(double2)) == hipSuccess && hipMalloc(&df, n * sizeof(double2)) == hipSuccess &&
+      hipMalloc(&di, n * sizeof(double2)) == hipSuccess &&
+      hipMemcpy(din, in, n * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+      gpe::launch_rowdft_selftest(din, df, di, nullptr) == hipSuccess &&
+      hipMemcpy(out_f, df, n * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess &&
+      hipMemcpy(out_i, di, n * sizeof(double2), hipMemcpyDeviceToHost) == hipSuccess)
     rc = GPE_OK;
   (void)hipFree(din);
   (void)hipFree(df);
   (void)hipFree(di);
-  (void)hipFree(dtw);
   return rc;
 }
 

@@ -124,7 +124,7 @@ struct gpe_handle {
   int var[3] = {0, 0, 0};
   // y-split 3D step (ny = 256): potential in the x sweeps' tile order, kinetic factor
   // per z pencil in the split's spectral order; v_gen counts writes of V[0]
-  bool ysplit_off = true;       // on with GPE_YSPLIT=1
+  bool ysplit_off = false;      // GPE_YSPLIT=0 turns the two-sweep step off
   double* Vp = nullptr;
   double2* kcolB = nullptr;
   double2* twy = nullptr;       // 256-point twiddles of the y axis
@@ -315,13 +315,14 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
   HIPCHK(h, hipMemcpyAsync(kax, axv.data(), axv.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
   std::vector<double2> colB;
   if (kcolB && h->ndims == 3 && h->ny == 256) {
-    // y-split step: the z pencil at y position l + 16 s holds ky index
-    // bitrev4(s) + 16 bitrev4(l) (16 x 16 split, both factors in bit-reversed order)
+    // y-split step: entry k2 + 16 s of a z tile (y block s) multiplies ky index
+    // bitrev4(s) + 16 k2 (the x sweep leaves its factor in bit-reversed order; the z
+    // sweep's factor over a runs in registers in natural order)
     auto br4 = [](int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); };
     colB.resize(col.size());
     for (int ix = 0; ix < h->nx; ++ix)
       for (int pos = 0; pos < 256; ++pos) {
-        const int ky = br4(pos >> 4) + 16 * br4(pos & 15);
+        const int ky = br4(pos >> 4) + 16 * (pos & 15);
         colB[(size_t)pos + 256 * (size_t)ix] = col[(size_t)ky + 256 * (size_t)ix];
       }
     HIPCHK(h, hipMemcpyAsync(kcolB, colB.data(), colB.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
@@ -402,7 +403,7 @@ inline int ysplit_prepare(gpe_handle* h) {
   if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
   if (h->vp_gen != h->v_gen) {
     HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           1u, h->stream));
+                                           (unsigned)GPE_YSPLIT_NA, h->stream));
     h->vp_gen = h->v_gen;
   }
   return GPE_OK;
@@ -477,7 +478,7 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
   int rc;
   if (h->vp_gen != h->v_gen) {
     HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           1u, h->stream));
+                                           (unsigned)GPE_YSPLIT_NA, h->stream));
     h->vp_gen = h->v_gen;
   }
   SweepParams px = ysplit_params(h);

@@ -1,15 +1,12 @@
-// DFT of length 16 across the 16 lanes of a DPP row (lanes 16q .. 16q+15 of a wave).
+// 16-point factors of the y transform of the two-sweep 3D step.
 //
 // The 3D sweeps of a 256-point y axis split the y transform as 256 = 16 x 16
-// (y = a + 16 b): one 16-point factor runs inside the x sweep and the other inside
-// the z sweep.  In both, the 16 values a factor combines sit in the 16 lanes of one
-// row, one per lane (one value per register m), so the factor is a radix-2 network
-// across lanes: partner values arrive by DPP lane permutations (no LDS traffic).
-//
-// Forward: decimation in frequency, lane r holds x_r on entry and X_{bitrev4(r)} on
-// exit, X_k = sum_r x_r W16^{rk}, W16 = e^{-2 pi i/16}.  Inverse: decimation in
-// time with conjugate factors, lane r holds X_{bitrev4(r)} on entry and
-// x_r = sum_k X_k W16^{-rk} (no 1/16) on exit, so inverse(forward(x)) = 16 x.
+// (y = a + 16 b): the factor over b runs inside the x sweep and the factor over a
+// inside the z sweep.  In both, the 16 values a factor combines belong to 16
+// neighbouring pencils of the work-group's tile.  The tile moves through LDS from its
+// pencil layout to a row layout in which one thread holds the 16 values, and the
+// factor is a 16-point DFT in registers (DFT<16>, fft_dev.h); a second exchange
+// moves it back.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,107 +14,43 @@
 
 namespace gpe {
 
-// DPP lane permutation of a double within each row (all rows and banks enabled).
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// Value of lane r ^ H of the same row.
-template <int H> __device__ __forceinline__ double row_xor(double x);
-template <> __device__ __forceinline__ double row_xor<1>(double x) { return dpp_d<0xB1>(x); }   // quad_perm 1,0,3,2
-template <> __device__ __forceinline__ double row_xor<2>(double x) { return dpp_d<0x4E>(x); }   // quad_perm 2,3,0,1
-// row_half_mirror (r ^ 7), then quad_perm 3,2,1,0 (r ^ 3): r ^ 4
-template <> __device__ __forceinline__ double row_xor<4>(double x) { return dpp_d<0x1B>(dpp_d<0x141>(x)); }
-template <> __device__ __forceinline__ double row_xor<8>(double x) { return dpp_d<0x128>(x); }  // row_ror 8
-
-template <int H>
-__device__ __forceinline__ double2 row_xor2(double2 v) {
-  return make_double2(row_xor<H>(v.x), row_xor<H>(v.y));
-}
-
 __device__ __forceinline__ int bitrev4(int r) {
   return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3);
 }
 
-// Per-lane factors of the three twiddled stages (half sizes 8, 4, 2): on the upper
-// lane of a pair, W_{2H}^{r mod H} = W16^{(r mod H) 8/H}; 1 on the lower lane.
-// tw holds W16^q at tw[q * stride16] (e.g. a staged twiddle table in LDS).
-// HOLD = true keeps the three factors in registers; false reads each at its stage.
-template <bool HOLD>
-struct RowFactorsT {
-  const double2* tw;
-  int r, stride16;
-  double2 wh[3];
-  __device__ __forceinline__ void init(int lane, const double2* table, int stride) {
-    tw = table;
-    r = lane;
-    stride16 = stride;
-    if (HOLD) {
-      wh[0] = load<8>();
-      wh[1] = load<4>();
-      wh[2] = load<2>();
+// A tile of C pencils of length N (256 threads, T = N/16 threads per pencil, 16
+// elements each) is G = C/16 blocks of 16 pencils.  It moves between the canonical
+// pencil layout -- thread (c = tid % C, t = tid / C) holds element t + T m of pencil
+// c = 16 g + a in v[m] -- and the row layout -- thread q = tid holds element z of
+// pencil 16 g + a in v[a], q = G z + g -- through LDS, real and imaginary parts in
+// turn (C T threads).  Slot of row q, pencil a: 16 q + (a ^ h(q)), h(q) = (q mod 16) ^ (q / 16 mod
+// 2).  Every 16 lanes then touch 16 distinct 8-byte bank pairs and every 32 lanes 32:
+// the pencil-layout writes cover the 16 values of a for one row q (two rows of
+// opposite parity per 32 lanes: two t for G = 1, two g otherwise), the row-layout
+// reads consecutive rows q, whose (q mod 2, h(q)) are distinct over 32 rows.  `lds`
+// holds C N doubles.
+__device__ __forceinline__ int row_slot(int a, int q) { return 16 * q + (a ^ ((q & 15) ^ ((q >> 4) & 1))); }
+
+template <int N, int C, bool TO_ROW>
+__device__ __forceinline__ void row_exchange16(double2 (&v)[16], double* lds) {
+  constexpr int G = C / 16, T = N / 16;
+  static_assert(C % 16 == 0 && (C * T) % 64 == 0, "whole blocks of 16 pencils, whole wavefronts");
+  const int tid = threadIdx.x, c = tid % C, t = tid / C, g = c / 16, a = c % 16;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int slot = TO_ROW ? row_slot(a, (t + T * m) * G + g) : row_slot(m, tid);
+      lds[slot] = half ? v[m].y : v[m].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const double val = lds[TO_ROW ? row_slot(m, tid) : row_slot(a, (t + T * m) * G + g)];
+      if (half) v[m].y = val; else v[m].x = val;
     }
   }
-  template <int H>
-  __device__ __forceinline__ double2 load() const {
-    return (r & H) ? tw[((r % H) * (8 / H)) * stride16] : make_double2(1.0, 0.0);
-  }
-  template <int H>
-  __device__ __forceinline__ double2 w() const {
-    if (HOLD) return wh[H == 8 ? 0 : H == 4 ? 1 : 2];
-    return load<H>();
-  }
-  // +1 on the lower lane of the pair at half size H, -1 on the upper
-  template <int H>
-  __device__ __forceinline__ double s() const { return (r & H) ? -1.0 : 1.0; }
-};
-using RowFactors = RowFactorsT<false>;
-
-// Decimation-in-frequency stage of half size H: lower lane a + b, upper lane (a - b) W.
-template <int E, int H, class RF>
-__device__ __forceinline__ void row_dif_stage(double2 (&v)[E], const RF& f) {
-  const double sg = f.template s<H>();
-  const double2 w = H > 1 ? f.template w<H>() : make_double2(1.0, 0.0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const double2 p = row_xor2<H>(v[m]);
-    const double2 d = make_double2(fma(sg, v[m].x, p.x), fma(sg, v[m].y, p.y));
-    v[m] = (H > 1) ? cmul(d, w) : d;
-  }
-}
-
-// Decimation-in-time stage of half size H with conjugate factors: the upper lane's
-// value times conj(W) is exchanged; lower lane a + b conj(W), upper lane a - b conj(W).
-template <int E, int H, class RF>
-__device__ __forceinline__ void row_dit_stage(double2 (&v)[E], const RF& f) {
-  const double sg = f.template s<H>();
-  const double2 w = H > 1 ? f.template w<H>() : make_double2(1.0, 0.0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const double2 y = (H > 1) ? cmulc(v[m], w) : v[m];
-    const double2 p = row_xor2<H>(y);
-    v[m] = make_double2(fma(sg, y.x, p.x), fma(sg, y.y, p.y));
-  }
-}
-
-template <int E, class RF>
-__device__ __forceinline__ void row_dft16_fwd(double2 (&v)[E], const RF& f) {
-  row_dif_stage<E, 8, RF>(v, f);
-  row_dif_stage<E, 4, RF>(v, f);
-  row_dif_stage<E, 2, RF>(v, f);
-  row_dif_stage<E, 1, RF>(v, f);
-}
-
-template <int E, class RF>
-__device__ __forceinline__ void row_dft16_inv(double2 (&v)[E], const RF& f) {
-  row_dit_stage<E, 1, RF>(v, f);
-  row_dit_stage<E, 2, RF>(v, f);
-  row_dit_stage<E, 4, RF>(v, f);
-  row_dit_stage<E, 8, RF>(v, f);
 }
 
 }  // namespace gpe

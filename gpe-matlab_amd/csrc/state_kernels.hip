@@ -212,16 +212,17 @@ hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, 
   }
 }
 
-// Potential in the tile order of the y-split x sweeps (tiles of na values of a):
-// one thread per output element.
+// Potential in the row order of the y-split x sweeps (tiles of na values of a):
+// Vp[(((z tpp + ap) 16 + b) nx + x) na + g] = V(y = na ap + g + 16 b, x, z), one thread
+// per output element.
 __global__ void __launch_bounds__(256) ysplit_permute_v_kernel(const double* __restrict__ V, double* __restrict__ Vp,
                                                                unsigned nx, unsigned na, unsigned total) {
   const unsigned i = blockIdx.x * 256u + threadIdx.x;
   if (i >= total) return;
-  const unsigned C = 16u * na, tpp = 16u / na;
-  const unsigned c = i % C, x = (i / C) % nx, tile = (i / C) / nx;   // tile = z*tpp + ap
+  const unsigned tpp = 16u / na;
+  const unsigned g = i % na, x = (i / na) % nx, b = (i / (na * nx)) % 16u, tile = i / (16u * na * nx);
   const unsigned ap = tile % tpp, z = tile / tpp;
-  const unsigned y = na * ap + (c >> 4) + 16u * (c & 15u);
+  const unsigned y = na * ap + g + 16u * b;
   Vp[i] = V[y + 256u * x + 256u * nx * z];
 }
 
@@ -231,28 +232,32 @@ hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, uns
   return hipGetLastError();
 }
 
-// One wavefront: value m of lane l is in[l + 64 m] (m < 2); out_f = forward row DFT,
-// out_i = inverse row DFT of the same input.
-__global__ void __launch_bounds__(64) rowdft_selftest_kernel(const double2* in, double2* out_f, double2* out_i,
-                                                             const double2* tw256) {
-  const int l = threadIdx.x;
-  RowFactors rf;
-  rf.init(l & 15, tw256, 16);
-  double2 f[2], g[2];
+// One tile of 16 pencils x 256 points (entry a + 16 z is point z of pencil a), loaded in
+// the pencil layout of the sweeps; the 16-point DFT over a (forward into out_f,
+// inverse without 1/16 into out_i) runs in the row layout, and the results return
+// through the reverse exchange.
+__global__ void __launch_bounds__(256) rowdft_selftest_kernel(const double2* in, double2* out_f, double2* out_i) {
+  __shared__ double lds[4096];
+  const int tid = threadIdx.x, c = tid & 15, t = tid >> 4;
+  double2 f[16], g[16];
 #pragma unroll
-  for (int m = 0; m < 2; ++m) f[m] = g[m] = in[l + 64 * m];
-  row_dft16_fwd<2>(f, rf);
-  row_dft16_inv<2>(g, rf);
+  for (int m = 0; m < 16; ++m) f[m] = in[c + 16 * (t + 16 * m)];
+  row_exchange16<256, 16, true>(f, lds);
 #pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    out_f[l + 64 * m] = f[m];
-    out_i[l + 64 * m] = g[m];
+  for (int m = 0; m < 16; ++m) g[m] = f[m];
+  DFT<16, -1>::run(f);
+  DFT<16, +1>::run(g);
+  row_exchange16<256, 16, false>(f, lds);
+  row_exchange16<256, 16, false>(g, lds);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    out_f[c + 16 * (t + 16 * m)] = f[m];
+    out_i[c + 16 * (t + 16 * m)] = g[m];
   }
 }
 
-hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, const double2* tw256,
-                                  hipStream_t s) {
-  hipLaunchKernelGGL(rowdft_selftest_kernel, dim3(1), dim3(64), 0, s, in, out_f, out_i, tw256);
+hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, hipStream_t s) {
+  hipLaunchKernelGGL(rowdft_selftest_kernel, dim3(1), dim3(256), 0, s, in, out_f, out_i);
   return hipGetLastError();
 }
 

@@ -161,9 +161,12 @@ hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_
 hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, int* nblocks);
 bool ysplit_x_supported(int n);
 // Vp[(z*(16/na) + ap)*16 na*nx + x*16 na + c] = V[(na ap + c/16 + 16 (c%16)) + 256 x + 256 nx z]
+// values of a per x-sweep tile of the y-split step (row order of the permuted potential)
+#ifndef GPE_YSPLIT_NA
+#define GPE_YSPLIT_NA 1
+#endif
 hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na,
                                    hipStream_t s);
-hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, const double2* tw256,
-                                  hipStream_t s);
+hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, hipStream_t s);
 
 }  // namespace gpe

@@ -154,19 +154,25 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
     for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_YKIN) {
-    // 16 adjacent pencils hold one y block (a = 0..15) in the 16 lanes of a row
-    // (C is a multiple of 16, lanes are consecutive pencils): the y factor over a runs
-    // across the row before the z transform and back after it
-    static_assert(COLFAST && C % 16 == 0, "y blocks must fill DPP rows");
-    RowFactorsT<true> rf;
-    rf.init(c & 15, twl, N / 16);   // W16^q = e^{-2 pi i q (N/16)/N}, from the staged twiddles
-    const double2 kc = p.kcol[cg];
-    row_dft16_fwd<E>(v, rf);
+    // Every 16 consecutive pencils of the tile are one y block (a = 0..15 of
+    // y = a + 16 s).  z transform in the pencil layout, then the tile moves to the row
+    // layout (thread q: kz index q / G of the 16 pencils of block g = q % G), where
+    // the y factor over a is a 16-point DFT in registers (natural order k2), the
+    // kinetic factor of ky = bitrev4(s) + 16 k2 (kcol entry k2 of the block) and kz
+    // applies, and the inverse factor runs; back in the pencil layout the inverse z
+    // transform follows.
+    constexpr int G = C / 16;
+    static_assert(COLFAST && C % 16 == 0 && E == 16, "blocks of 16 pencils, 16 elements per thread");
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    row_exchange16<N, C, true>(v, lds);
+    DFT<16, -1>::run(v);
+    const double2 kz = twl[N + tid / G];
+    const double2* __restrict__ kc = p.kcol + tile * (unsigned)C + 16u * (unsigned)(tid % G);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
+    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc[m], kz), v[m]);
+    DFT<16, +1>::run(v);
+    row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
-    row_dft16_inv<E>(v, rf);
   } else if constexpr (SEQ == S_KEX || SEQ == S_KE) {
     if constexpr (SEQ == S_KEX) {
       if (active) {

@@ -2,16 +2,17 @@
 // S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65).
 //
 // The y axis has 256 points, y = a + 16 b.  A work-group takes the tile of one z
-// plane with a in {2 ap, 2 ap + 1} and every b: 32 x-pencils, pencil c <-> (a = 2 ap
-// + c/16, b = c % 16), T = nx/E threads each (tid = t*32 + c).  Each memory
-// instruction of a wave then reads 32-byte pieces (the two a of one b) of 32 lines,
-// and the 8 tiles of a plane run on one XCD at about the same time, so its L2 serves
-// the other pieces of each line.  The 16 pencils with one a fill a DPP row, so the
-// y factor over b (radix-2 across lanes, rowdft_dev.h) needs no data movement; the
-// twiddle W256^{a k1} of the 16 x 16 split is applied here.
+// plane with NA values of a (a = NA ap + g) and every b: C = 16 NA x-pencils, pencil
+// c <-> (a = NA ap + c / 16, b slot r = c % 16), T = nx/16 threads each, 16 elements
+// per thread.  The x transforms run in that pencil layout (tid = t C + c); the tile
+// then moves through LDS to the row layout (thread q = NA x + g holds the 16 b slots
+// of one (a, x), row_exchange16), where the y factor over b is a 16-point DFT in
+// registers and the phase step applies.  Between the sweeps the b slot r holds the
+// spectral index k1 = bitrev4(r), times the twiddle W256^{a k1} of the 16 x 16 split;
+// the z sweep applies the factor over a.
 //
-// The potential is read in tile order (launch_ysplit_permute_v): one contiguous
-// 256-byte row per pencil index m and pair of lanes t.
+// The potential is read in row order (launch_ysplit_permute_v): for each b, the NA nx
+// values of the tile's (x, g) rows are contiguous.
 #include <type_traits>
 
 #include "rowdft_dev.h"
@@ -40,21 +41,17 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
   return r;
 }
 
-// PV 0: the plan of the phase sweeps (8.8.4 for 256: 8 elements per thread);
-// PV 1: Plan<N> (16.16 for 256: 16 elements per thread, half the threads).
-template <int N, int PV>
-using YPlan = typename std::conditional<PV == 0, typename SweepPlan<N, S_MID, 0>::type, Plan<N>>::type;
+constexpr int br4c(int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); }
 
-// NA = number of a values per tile (1: 16 pencils, 16-byte pieces; 2: 32 pencils,
-// 32-byte pieces).  Tiles per z plane: 16 / NA.
-template <int N, int NA, int PV>
-constexpr int ythreads() { return 16 * NA * YPlan<N, PV>::T; }
+template <int N, int NA>
+constexpr int ythreads() { return 16 * NA * (N / 16); }
 
-template <int N, int SEQ, int NA, int PV>
-__global__ void __launch_bounds__((ythreads<N, NA, PV>())) ysplit_x_kernel(const SweepParams p) {
-  using PL = YPlan<N, PV>;
+template <int N, int SEQ, int NA>
+__global__ void __launch_bounds__((ythreads<N, NA>())) __attribute__((amdgpu_waves_per_eu(2)))
+ysplit_x_kernel(const SweepParams p) {
+  using PL = Plan<N>;
   constexpr int T = PL::T, E = PL::E, C = 16 * NA, NT = C * T, NW = NT / 64, TPP = 16 / NA;
-  static_assert(NT % 64 == 0, "whole wavefronts");
+  static_assert(E == 16 && NT % 64 == 0, "16 elements per thread, whole wavefronts");
   __shared__ double lds[Lds<N, C, true>::SIZE];
   __shared__ double2 twl[N];
   __shared__ double2 twy[256];
@@ -67,79 +64,81 @@ __global__ void __launch_bounds__((ythreads<N, NA, PV>())) ysplit_x_kernel(const
   // the TPP tiles of a z plane go to one XCD (group b runs on XCD b % 8); gridDim = TPP nz
   const unsigned bi = blockIdx.x, xcd = bi & 7u, s8 = bi >> 3;
   const unsigned ap = s8 % (unsigned)TPP, z = (s8 / (unsigned)TPP) * 8u + xcd;
-  const int c = tid % C, t = tid / C;
-  const int r = c & 15;                        // lane in the DPP row = b
-  const int a = NA * (int)ap + (c >> 4);
   const unsigned ny = p.gny, nx = p.gnx;
-  const unsigned base = (unsigned)(a + 16 * r) + ny * (unsigned)t + ny * nx * z;
+  // pencil layout
+  const int c = tid % C, t = tid / C;
+  const unsigned base = (unsigned)(NA * (int)ap + (c >> 4) + 16 * (c & 15)) + ny * (unsigned)t + ny * nx * z;
   const unsigned step = (unsigned)T * ny;
-  const unsigned vbase = (z * (unsigned)TPP + ap) * (unsigned)C * nx + (unsigned)t * (unsigned)C + (unsigned)c;
-  const unsigned vstep = (unsigned)T * (unsigned)C;
+  // row layout: a = NA ap + g of row q = tid
+  const int ar = NA * (int)ap + tid % NA;
+  const unsigned vrow = (z * (unsigned)TPP + ap) * 16u * (unsigned)NT + (unsigned)tid;
 
   double2 v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = p.psi[base + (unsigned)m * step];
-  double vv[E];
+  // the potential is loaded with the state, before the transform: at two waves per SIMD
+  // nothing else would hide its latency at the phase
+  double vv[16];
 #pragma unroll
-  for (int m = 0; m < E; ++m) vv[m] = p.V[vbase + (unsigned)m * vstep];
+  for (int b = 0; b < 16; ++b) vv[b] = p.V[vrow + (unsigned)b * (unsigned)NT];
 
-  RowFactors rf;
-  rf.init(r, twy, 16);
-  // W256^{a k1}, k1 = bitrev4(b lane) after the forward factor (read where used)
-  const int ia = a * bitrev4(r);
   const bool has_re = p.dr != 0.0;
   const double mu = p.st->mu;
+  double2 u[16];   // row layout, b natural (real space) or k1 natural (spectral)
 
   if constexpr (SEQ == S_YOPEN) {
-    // solve_split.m:55-56 with the pending renormalisation (solve_split.m:72-73)
+    // real-space state: slot r holds b = r.  solve_split.m:55-56 with the pending
+    // renormalisation (solve_split.m:72-73)
+    row_exchange16<N, C, true>(v, lds);
     const double sc = p.st->scale;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double2 x = cscale(v[m], sc);
-      const double aa = -(vv[m] - mu) - p.g * cabs2(x);
-      v[m] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);
+    for (int b = 0; b < 16; ++b) {
+      const double2 x = cscale(v[b], sc);
+      const double aa = -(vv[b] - mu) - p.g * cabs2(x);
+      u[b] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);
     }
   } else {
-    // the previous x sweep left the y factor over b applied: undo it
-    {
-      const double2 wa = twy[ia];
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cmulc(v[m], wa);
-    }
-    row_dft16_inv<E>(v, rf);
     fft_pencil<PL, +1, C, true>(v, t, c, lds, twl);
+    row_exchange16<N, C, true>(v, lds);
+    // undo the split twiddle and the y factor over b left by the previous x sweep
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double vm = vv[m] - mu;
-      const double aa = -vm - p.g * cabs2(v[m]);                  // solve_split.m:60
-      double2 x = mul_cexp(v[m], aa * p.dr, aa * p.di, has_re);    // solve_split.m:61
+    for (int k1 = 0; k1 < 16; ++k1) u[k1] = cmulc(v[br4c(k1)], twy[ar * k1]);
+    DFT<16, +1>::run(u);
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const double vm = vv[b] - mu;
+      const double aa = -vm - p.g * cabs2(u[b]);                  // solve_split.m:60
+      double2 x = mul_cexp(u[b], aa * p.dr, aa * p.di, has_re);    // solve_split.m:61
       if constexpr (SEQ == S_YCLOSEM) {
         const double bb = vm + p.g * cabs2(x);                       // solve_split.m:63
         x = mul_cexp(x, bb * p.dr * 0.5, bb * p.di * 0.5, has_re);   // solve_split.m:64
       }
-      v[m] = x;
+      u[b] = x;
     }
   }
   if constexpr (SEQ == S_YOPEN || SEQ == S_YMID) {
+    DFT<16, -1>::run(u);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * k1]);
+    row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
-    row_dft16_fwd<E>(v, rf);
-    const double2 wa = twy[ia];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], wa);
-  }
-#pragma unroll
-  for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
-
-  if constexpr (SEQ == S_YCLOSEM) {
-    // block-end reductions of the stored real-space state (solve_split.m:65-77)
+    for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+  } else {
+    // block-end reductions of the real-space state (solve_split.m:65-77), which is
+    // stored with slot r = b
     double accA = 0.0, accB = 0.0, accC = 0.0, accK = 0.0;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double n = cabs2(v[m]);
+    for (int b = 0; b < 16; ++b) {
+      const double n = cabs2(u[b]);
       accA += n;
-      accB += vv[m] * n;
+      accB += vv[b] * n;
       accC += n * n;
+      v[b] = u[b];
     }
+    row_exchange16<N, C, false>(v, lds);
+#pragma unroll
+    for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
     fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
@@ -162,25 +161,23 @@ __global__ void __launch_bounds__((ythreads<N, NA, PV>())) ysplit_x_kernel(const
 
 // Templated on the length as well: this file is compiled once per length, and a helper
 // named only by NA would be one merged symbol across the objects.
-template <int N, int NA, int PV>
+template <int N, int NA>
 static hipError_t launch_na(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
-  const dim3 grid((16u / NA) * p.gnz), block(ythreads<N, NA, PV>());
+  const dim3 grid((16u / NA) * p.gnz), block(ythreads<N, NA>());
   switch (seq) {
-    case S_YOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YOPEN, NA, PV>), grid, block, 0, s, p); break;
-    case S_YMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YMID, NA, PV>), grid, block, 0, s, p); break;
-    case S_YCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEM, NA, PV>), grid, block, 0, s, p); break;
+    case S_YOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YOPEN, NA>), grid, block, 0, s, p); break;
+    case S_YMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YMID, NA>), grid, block, 0, s, p); break;
+    case S_YCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEM, NA>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   if (nblocks) *nblocks = (int)grid.x;
   return hipGetLastError();
 }
 
-// One value of a per tile (16 pencils) with Plan<N> (16 elements per thread at 256):
-// the fastest of the measured tile/plan combinations.
 hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
   constexpr int N = GPE_N;
   if (p.gny != 256u || p.gnx != (unsigned)N || p.gnz % 8u != 0u || p.gnz == 0u) return hipErrorInvalidValue;
-  return launch_na<N, 1, 1>(seq, p, s, nblocks);
+  return launch_na<N, GPE_YSPLIT_NA>(seq, p, s, nblocks);
 }
 
 }  // namespace gpe

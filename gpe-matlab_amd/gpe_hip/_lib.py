@@ -570,13 +570,13 @@ def wisdom_import(text):
 
 
 def selftest_rowdft(x):
-    """Forward and inverse 16-point DFT across DPP rows (test hook): x holds 128
-    complex values, entry l + 64 m is value m of lane l."""
+    """Forward and inverse 16-point DFT over the pencils of one 16 x 256 tile (test
+    hook): x holds 4096 complex values, entry a + 16 z is point z of pencil a."""
     lib = load()
     x = np.ascontiguousarray(x, dtype=np.complex128)
-    assert x.size == 128
-    f = np.empty(128, dtype=np.complex128)
-    i = np.empty(128, dtype=np.complex128)
+    assert x.size == 4096
+    f = np.empty(4096, dtype=np.complex128)
+    i = np.empty(4096, dtype=np.complex128)
     check(lib.gpe_selftest_rowdft(x.view(np.float64).ctypes.data_as(_DP), f.view(np.float64).ctypes.data_as(_DP),
                                   i.view(np.float64).ctypes.data_as(_DP)), None, "gpe_selftest_rowdft")
     return f, i

@@ -341,12 +341,11 @@ int gpe_profile_read(gpe_handle* h, double* ms, int* count, int n);
  * sweep kernels use, for checking it against a host math library. */
 int gpe_selftest_phase(int n, const double* er, const double* ei, double* out);
 
-/* Self-test hook (no reference counterpart): the 16-point DFT across the 16 lanes
- * of a DPP row that the 3D sweeps of a 256-point y axis use.  in, out_f, out_i hold
- * 128 interleaved complex values; value m of lane l (l < 64, m < 2) is entry l + 64m,
- * lanes 16q..16q+15 form row q.  out_f: forward, lane r receives
- * X_{bitrev4(r)} = sum_j x_j e^{-2 pi i j bitrev4(r)/16}; out_i: inverse, lane r
- * receives sum_k in_{bitrev4(k)} e^{+2 pi i r k/16} (no 1/16). */
+/* Self-test hook (no reference counterpart): the 16-point factor of the two-sweep
+ * 3D step on one tile of 16 pencils x 256 points, moved from the sweeps' pencil layout
+ * to the row layout and back.  in, out_f, out_i hold 4096 interleaved complex values,
+ * entry a + 16 z = point z of pencil a.  out_f[k + 16 z] = sum_a in[a + 16 z]
+ * e^{-2 pi i a k/16}; out_i the same with e^{+2 pi i a k/16} (no 1/16). */
 int gpe_selftest_rowdft(const double* in, double* out_f, double* out_i);
 
 /* Library/device information for diagnostics: writes a NUL-terminated string. */

@@ -30,14 +30,18 @@ def _c2_task(mod, **props):
     return t, psi0, grid
 
 
-def test_c2_full_size_parity_10_steps():
+@pytest.mark.parametrize("schedule", ["two-sweep", "four-sweep"])
+def test_c2_full_size_parity_10_steps(schedule, monkeypatch):
+    # the two-sweep step is the default for the 256-point y axis; GPE_YSPLIT=0 keeps the
+    # four sweeps (read when the handle is created)
     import gpe_hip
+    monkeypatch.setenv("GPE_YSPLIT", "1" if schedule == "two-sweep" else "0")
     tg, _, _ = _c2_task(gpe_hip)
     to, _, _ = _c2_task(oracle)
     pg = tg.solve_split(0.002, 10, 1)
     po = to.solve_split(0.002, 10, 1)
     err = rel_l2(pg, po)
-    print(f"256^3 gamma=0.01, 10 steps: rel-L2 vs oracle {err:.3e}", flush=True)
+    print(f"256^3 gamma=0.01, 10 steps, {schedule}: rel-L2 vs oracle {err:.3e}", flush=True)
     assert err < 1e-11
     assert np.allclose(tg.history["mu"], to.history["mu"], rtol=1e-11, atol=0)
     assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)

@@ -1,10 +1,10 @@
 """The two-sweep 3D step of grids with a 256-point y axis (y = a + 16 b split).
 
 The y transform is factored 16 x 16: the factor over b runs inside the x sweep,
-the factor over a inside the z sweep, each as a radix-2 network across the 16 lanes
-of a DPP row.  Checked here: the row DFT against numpy's FFT (in its bit-reversed
-order), and GPEtask.solve_split on such grids against the FP64 oracle (rel-L2 <=
-1e-11) and against the four-sweep path of the same library (GPE_YSPLIT=0).
+the factor over a inside the z sweep, each as a 16-point DFT in registers after an
+LDS exchange of the tile.  Checked here: the exchange + DFT against numpy's FFT, and
+GPEtask.solve_split on such grids against the FP64 oracle (rel-L2 <= 1e-11) and
+against the four-sweep path of the same library (GPE_YSPLIT=0).
 """
 import os
 
@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(autouse=True)
 def two_sweep_step():
-    """Handles created inside these tests run the two-sweep step (it is opt-in)."""
+    """Handles created inside these tests run the two-sweep step (the default, set
+    explicitly so the tests do not depend on the environment)."""
     old = os.environ.get("GPE_YSPLIT")
     os.environ["GPE_YSPLIT"] = "1"
     yield
@@ -29,31 +30,27 @@ def two_sweep_step():
         os.environ["GPE_YSPLIT"] = old
 
 TOL = 1e-11
-BR4 = np.array([int(f"{r:04b}"[::-1], 2) for r in range(16)])
 
 
 def test_row_dft16_matches_numpy():
+    """The 16-point factor through the pencil -> row -> pencil exchange of one tile
+    (gpe_selftest_rowdft) against numpy's FFT over the pencil index."""
     from gpe_hip import _lib
     rng = np.random.Generator(np.random.PCG64(11))
-    x = rng.standard_normal(128) + 1j * rng.standard_normal(128)
+    x = rng.standard_normal(4096) + 1j * rng.standard_normal(4096)
     f, i = _lib.selftest_rowdft(x)
-    for m in range(2):
-        for q in range(4):
-            lanes = np.arange(16) + 16 * q + 64 * m
-            X = np.fft.fft(x[lanes])
-            np.testing.assert_allclose(f[lanes], X[BR4], rtol=0, atol=1e-13 * np.abs(X).max())
-            # inverse: lane r holds X_{bitrev4(r)} on entry
-            Y = np.empty(16, dtype=complex)
-            Y[BR4] = x[lanes]
-            y = np.fft.ifft(Y) * 16
-            np.testing.assert_allclose(i[lanes], y, rtol=0, atol=1e-13 * np.abs(y).max())
-    # a unit impulse in lane 3 of row 1: forward gives W16^{3 k} in bit-reversed order
-    e = np.zeros(128, dtype=complex)
-    e[16 + 3] = 1.0
+    X = x.reshape(256, 16)                       # [z, a]
+    F = np.fft.fft(X, axis=1)
+    Iv = np.fft.ifft(X, axis=1) * 16
+    np.testing.assert_allclose(f.reshape(256, 16), F, rtol=0, atol=1e-13 * np.abs(F).max())
+    np.testing.assert_allclose(i.reshape(256, 16), Iv, rtol=0, atol=1e-13 * np.abs(Iv).max())
+    # a unit impulse at pencil 3 of point 17: W16^{3 k} on that point, zero elsewhere
+    e = np.zeros(4096, dtype=complex)
+    e[3 + 16 * 17] = 1.0
     f, _ = _lib.selftest_rowdft(e)
-    k = BR4
-    np.testing.assert_allclose(f[16:32], np.exp(-2j * np.pi * ((3 * k) % 16) / 16), rtol=0, atol=1e-15)
-    assert np.all(f[:16] == 0) and np.all(f[32:] == 0)
+    k = np.arange(16)
+    np.testing.assert_allclose(f[16 * 17: 16 * 18], np.exp(-2j * np.pi * ((3 * k) % 16) / 16), rtol=0, atol=1e-15)
+    assert np.count_nonzero(f) == 16
 
 
 def _pair(n, xmax):
