@@ -45,15 +45,20 @@ STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 STEP_BYTES_3D_YSPLIT = 72  # two-sweep step: 2 x 32 B + 8 B of V
 
 
-def committed_traffic(n):
-    """HBM bytes per launch per sweep from the committed rocprofv3 counter summary."""
-    path = os.path.join(ROOT, "profiles", f"sweep{n}_c2_hbm_traffic.md")
+TRAFFIC_FILES = {"four": "sweep{n}_c2_hbm_traffic.md", "two": "ysplit{n}_c2_two_sweep.md"}
+
+
+def committed_traffic(n, schedule):
+    """HBM bytes per launch per sweep from the committed rocprofv3 counter summary of the
+    schedule (copy-calibrated FETCH_SIZE + WRITE_SIZE of the same workload)."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILES[schedule].format(n=n))
     if not os.path.exists(path):
-        return {}
+        return {}, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].strip().split(", "))}
-    return {}
+            vals = {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].strip().split(", "))}
+            return vals, os.path.relpath(path, ROOT)
+    return {}, None
 
 
 MIN_WARMUP = 100   # untimed steps before the timed region, at least
@@ -235,14 +240,14 @@ def main():
     for k in kernels:
         log(f"  {k['kernel']:8s} x{k['launches']:3d} avg {k['avg_ms'] * 1e3:8.1f} us  {k['alg_GBps']:7.1f} GB/s alg")
     dom = kernels[0]
-    traffic = committed_traffic(args.n).get(dom["kernel"])
+    ysplit_on = any(SEQ_NAMES[seq] == "ymid" for (seq, _ax) in prof)
+    tvals, tsrc = committed_traffic(args.n, "two" if ysplit_on else "four")
+    traffic = tvals.get(dom["kernel"])
     roofline = {"bound": "hbm", "achieved": round(dom["alg_GBps"], 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                 "frac": round(dom["alg_GBps"] / PEAK_HBM_GBPS, 4), "traffic": traffic, "kernel": dom["kernel"],
                 "alg_bytes_per_launch": dom["alg_bytes"],
-                "traffic_source": "profiles/sweep256_c2_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
-                if traffic else None}
+                "traffic_source": f"{tsrc} (FETCH_SIZE/WRITE_SIZE, copy-calibrated, committed)" if traffic else None}
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
-    ysplit_on = any(SEQ_NAMES[seq] == "ymid" for (seq, _ax) in prof)
 
     # ------------------------------------------------- host-inclusive drop-in call
     # The same steps through the drop-in call from host arrays, as a MATLAB-side caller
