@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "gpe_hip", "libgpehip.so")
 LENGTHS = (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)
-SOURCES = ("state_kernels.hip", "tune.hip", "gpe_capi.hip", "spinor.hip", "adjoint.hip", "rabi_adjoint.hip", "itp2.hip", "vtd.hip", "sgpe.hip", "diag.hip", "batch.hip")
+SOURCES = ("state_kernels.hip", "tune.hip", "gpe_capi.hip", "spinor.hip", "adjoint.hip", "rabi_adjoint.hip", "itp2.hip", "vtd.hip", "sgpe.hip", "diag.hip", "batch.hip", "applyham.hip")
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
 
 

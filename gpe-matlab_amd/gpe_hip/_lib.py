@@ -97,6 +97,7 @@ SIGNATURES = {
     "gpe_split_outer": [_P, C.c_int, _DP, _DP, _DP],
     "gpe_split_block": [_P, _D, _DP, _DP],
     "gpe_measure": [_P, _DP],
+    "gpe_applyham": [_P, _DP, _DP, C.c_int],
     "gpe_itp": [_P, C.POINTER(ItpArgs), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_itp2": [_P, C.POINTER(Itp2Args), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_device_info": [C.c_char_p, C.c_int],
@@ -342,6 +343,20 @@ class Device:
         out = np.zeros(5)
         self.call("gpe_measure", dptr(out))
         return out
+
+    def applyham(self, phis, ncomp, h0_only=False):
+        """applyham / applyh0 of the ncomp arrays `phis` (None: the current state) as a
+        list of C-order arrays of the grid's shape."""
+        n = int(np.prod(self.shape))
+        out = np.empty(ncomp * n, dtype=np.complex128)
+        if phis is None:
+            src = None
+        else:
+            src = np.concatenate([np.asarray(p, dtype=np.complex128).reshape(self.shape).ravel(order="F")
+                                  for p in phis])
+        self.call("gpe_applyham", None if src is None else src.view(np.float64).ctypes.data_as(_DP),
+                  out.view(np.float64).ctypes.data_as(_DP), int(bool(h0_only)))
+        return [out[j * n:(j + 1) * n].reshape(self.shape, order="F") for j in range(ncomp)]
 
     def split_begin(self, ddt, niter_inner):
         muc = C.c_double()

@@ -165,6 +165,41 @@ class GPEtask:
         m = self._measure(phi, time)
         return (m[1] + m[3]) + self.g * m[2]
 
+    def applyham(self, phi=None, time=None):
+        """GPEtask.m:73-78 -- H phi = applyh0(phi) + g |phi|^2 phi as an array, on the device
+        (phi = None: current_state)."""
+        return self._apply(None if phi is None else [phi], time, False)[0]
+
+    def applyh0(self, phi=None, time=None):
+        """GPEtask.m:80-95 -- getVtotal(time) phi + lap(phi[, omega]) as an array, on the device."""
+        return self._apply(None if phi is None else [phi], time, True)[0]
+
+    def _potential_at(self, time):
+        self._device().set_potential(0, self.getVtotal(time))
+
+    def _apply(self, phis, time, h0):
+        """The Hamiltonian of the arrays `phis` (None: the current state) with the potential
+        of `time` (default current_time).  Inside user_callback the running solve_split's
+        state, potential and parameters are left as they were, as for the reductions."""
+        self._check_omega()
+        dev = self._device()
+        if self._resident(phis, time):
+            return dev.applyham(None, self._ncomp, h0)
+        if not self._in_callback:
+            self._set_params(self.Ntotal if self.Ntotal > 0 else 1.0)
+        if time is None:
+            time = self.current_time
+        self._potential_at(time)
+        if phis is None and not self._dev_state_valid:
+            if self.current_state is None:
+                raise ValueError("no current state")
+            self._upload(self.current_state)
+            self._dev_state_valid = True
+        res = dev.applyham(phis, self._ncomp, h0)
+        if self._in_callback and time != self.current_time:
+            self._potential_at(self.current_time)   # the running block's potential
+        return res
+
     # ------------------------------------------------------------------ diagnostics
     def detect_core(self, refine=0, zslice=None, comp=0):
         """lib/util/detect_core.m:1-66 on the resident state, without copying it to the host:
@@ -459,6 +494,26 @@ class GPEtaskSpinor(GPEtask):
         """GPEtaskSpinor.m:100-111 -- real <phi|H_{g/2} phi>, reduced on the device."""
         m = self._measure_sp(phi, time)
         return (m[1] + m[3]) + m[4] + 0.5 * m[2]
+
+    def applyham(self, phi=None, time=None):
+        """GPEtaskSpinor.m:77-86 -- [applyham1c(phi, 1), applyham1c(phi, 2)] as arrays, on the
+        device (phi = None: current_state)."""
+        return self._apply(None if phi is None else list(phi), time, False)
+
+    def applyham1c(self, phi, j, time=None):
+        """GPEtaskSpinor.m:55-76 -- component j (0-based here; 1-based in MATLAB) of H phi:
+        (V_j + sum_k g_jk |phi_k|^2) phi_j + lap(phi_j) + Omega phi_2 (j = 0) or
+        conj(Omega) phi_1 (j = 1)."""
+        return self.applyham(phi, time)[j]
+
+    def applyh0(self, phi, j, time=None):
+        """GPEtaskSpinor.m:45-54 -- lap(phi) + V_j phi for one array phi and component j
+        (0-based)."""
+        return self._apply([phi, phi], time, True)[j]
+
+    def _potential_at(self, time):
+        self._upload_potential(self.getVtotal(time))
+        self._upload_coupling()
 
     def _measure_sp(self, phi, time):
         self._check_omega()

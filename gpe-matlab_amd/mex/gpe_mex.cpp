@@ -294,6 +294,19 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     double m[5];
     check(gpe_measure(t.h, m), t.h, "gpe_measure");
     set_out(nlhs, plhs, 0, column(m, 5));
+  } else if (cmd == "applyham") {
+    // applyham / applyh0 as arrays (GPEtask.m:73-95, GPEtaskSpinor.m:45-86):
+    // H = gpe_mex('applyham', h, ncomp, phi, h0_only); phi = [] takes the current state.
+    // Two components: phi and H are [grid.size, 2], the component along the last dimension.
+    need(nrhs, 5, c);
+    Task& t = task_of(prhs[1]);
+    const int nc = as_int(prhs[2]);
+    if (nc != 1 && nc != 2) mexErrMsgIdAndTxt("gpe:args", "applyham: ncomp must be 1 or 2");
+    std::vector<double> tmp;
+    const double* in = mxIsEmpty(prhs[3]) ? nullptr : complexes(prhs[3], (size_t)nc * t.npts(), tmp, "phi");
+    mxArray* out = grid_complex(t, nc);
+    check(gpe_applyham(t.h, in, cdata(out), as_int(prhs[4])), t.h, "gpe_applyham");
+    set_out(nlhs, plhs, 0, out);
   } else if (cmd == "itp") {
     // groundstate_itp.m:20-112 after process_init_state:
     // [MU, MU2, EE, n_iter, dt_final, converged] = gpe_mex('itp', h, dt, eps, max_iter, min_iter, adjust, nargout)

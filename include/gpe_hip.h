@@ -135,6 +135,16 @@ int gpe_split_block(gpe_handle* h, double time2, double* muc, double* ncur);
  * (spinor: summed over components; out[4] = 2 Re sum conj(psi1) Omega psi2 w). */
 int gpe_measure(gpe_handle* h, double out[5]);
 
+/* applyham(phi) / applyh0(phi) as arrays (GPEtask.m:73-95; GPEtaskSpinor.m:45-86:
+ * applyham, applyham1c of each component, applyh0).  phi and out hold ncomp
+ * column-major complex arrays of npts values, component after component; phi = NULL
+ * applies the operator to the current state (with its pending renormalisation).
+ * h0_only = 0: lap(phi[, omega]) + V phi + the interaction (g|phi|^2 phi, or
+ * sum_k g_jk |phi_k|^2 phi_j) + the Rabi coupling (Omega phi2 / conj(Omega) phi1);
+ * h0_only = 1: lap(phi[, omega]) + V phi.  V is the step potential (gpe_set_potential).
+ * Work buffers are allocated for the call ((2 ncomp + 1) npts complex values). */
+int gpe_applyham(gpe_handle* h, const double* phi, double* out, int h0_only);
+
 /* groundstate_itp after process_init_state (lib/@GPEtask/groundstate_itp.m:20-112).
  * The initial state must be uploaded (gpe_set_state).  MU, MU2, EE are caller
  * arrays of max_iter entries (MU2/EE may be NULL when nargout < 3/4).  Returns

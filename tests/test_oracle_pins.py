@@ -547,3 +547,55 @@ def test_spinor_get_energy_uniform_closed_form():
     n1, n2 = abs(a) ** 2 * vol, abs(b) ** 2 * vol
     E = uniform_spinor_energy(gm, n1, n2, om, a, b, vol)
     assert abs(t.get_energy(phi) / E - 1) < 1e-13
+
+
+def _plane_wave(grid, idx):
+    ks = (grid.kx1[idx[0]], grid.ky1[idx[1]]) + ((grid.kz1[idx[2]],) if grid.ndims == 3 else ())
+    arg = ks[0] * grid.mesh.x + ks[1] * grid.mesh.y + (ks[2] * grid.mesh.z if grid.ndims == 3 else 0)
+    return np.exp(1j * arg), 0.5 * sum(k * k for k in ks)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_applyham_plane_wave_eigenvalue(dims):
+    """A plane wave A e^{ik.x} in a uniform potential c: applyh0 = (k^2/2 + c) psi and
+    applyham = (k^2/2 + c + g |A|^2) psi (GPEtask.m:73-95; the Nyquist index included)."""
+    grid = Grid2D(5, 16, 4, 12) if dims == 2 else Grid3D(5, 16, 4, 12, 3, 8)
+    idx = (3, -2, 4) if dims == 3 else (8, -5)
+    e, kk = _plane_wave(grid, idx)
+    A = 0.8 * np.exp(0.4j)
+    c = 0.35
+    t = GPEtask(grid, c + 0 * grid.mesh.x)
+    t.g = 1.3
+    psi = A * e
+    assert rel_l2(t.applyh0(psi), (kk + c) * psi) < 1e-13
+    assert rel_l2(t.applyham(psi), (kk + c + t.g * abs(A) ** 2) * psi) < 1e-13
+
+
+def test_spinor_applyham_plane_wave_closed_form():
+    """psi_j = c_j e^{ik.x}, uniform V_j, uniform complex Omega: component 1 couples
+    through Omega, component 2 through conj(Omega) (GPEtaskSpinor.m:55-76), and each takes
+    sum_k g_jk |c_k|^2 with the matrix read as g(j, k)."""
+    grid = Grid2D(5, 16, 4, 12)
+    e, kk = _plane_wave(grid, (2, 3))
+    t = GPEtaskSpinor(grid, [0.2 + 0 * grid.mesh.x, -0.1 + 0 * grid.mesh.x])
+    gm = np.array([[0.5, 0.45], [0.3, 0.6]])
+    t.g = gm
+    om = 0.3 * np.exp(0.9j)
+    t.coupling = om
+    a, b = 0.7 * np.exp(0.2j), 0.4 * np.exp(-1.1j)
+    h1, h2 = t.applyham([a * e, b * e])
+    n = np.array([abs(a) ** 2, abs(b) ** 2])
+    assert rel_l2(h1, ((kk + 0.2 + gm[0] @ n) * a + om * b) * e) < 1e-13
+    assert rel_l2(h2, ((kk - 0.1 + gm[1] @ n) * b + np.conj(om) * a) * e) < 1e-13
+
+
+@pytest.mark.parametrize("m", [1, -1])
+def test_rotating_applyh0_vortex_eigenvalue(m):
+    """(x + i m y) e^{-r^2/2} in V = r^2/2: applyh0 with a scalar omega gives
+    (2 + m omega) psi up to the grid's spectral error (GPEtask.m:85-89, grid2d/lap.m:8-9)."""
+    grid = oracle.grid2d(8, 64, 8, 64)
+    V = 0.5 * (grid.mesh.x ** 2 + grid.mesh.y ** 2)
+    psi = (grid.mesh.x + 1j * m * grid.mesh.y) * np.exp(-(grid.mesh.x ** 2 + grid.mesh.y ** 2) / 2)
+    t = GPEtask(grid, V)
+    t.omega = 0.4
+    assert rel_l2(t.applyh0(psi), (2 + m * 0.4) * psi) < 1e-10
