@@ -287,9 +287,11 @@ struct Lds {
 };
 
 // Restore the canonical layout after pass p via LDS (doubles, re and im staged
-// separately so the LDS footprint is N*C doubles).
+// separately so the LDS footprint is N*C doubles).  Every slot index is XOR-ed with the
+// lane's `sw` (0 for a plain sweep); a non-zero sw below 32 keeps the index inside its
+// 32-slot line, so it needs a footprint of whole lines.
 template <class PL, int p, int C, bool COLFAST>
-__device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, double* lds) {
+__device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, double* lds, unsigned sw = 0) {
   using L = Lds<PL::N, C, COLFAST>;
   constexpr int R = PL::radix(p);
   constexpr int B = PL::E / R;
@@ -301,12 +303,12 @@ __device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, doub
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int idx = pass_out_index<PL, p>(t, b, r);
-        lds[L::at(c, idx)] = half ? v[b + r * B].y : v[b + r * B].x;
+        lds[L::at(c, idx) ^ sw] = half ? v[b + r * B].y : v[b + r * B].x;
       }
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < PL::E; ++m) {
-      const double val = lds[L::at(c, t + PL::T * m)];
+      const double val = lds[L::at(c, t + PL::T * m) ^ sw];
       if (half) v[m].y = val; else v[m].x = val;
     }
   }
@@ -315,18 +317,18 @@ __device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, doub
 // Full transform of one pencil held in canonical layout.
 template <class PL, int DIR, int C, bool COLFAST>
 __device__ __forceinline__ void fft_pencil(double2 (&v)[PL::E], int t, int c, double* lds,
-                                           const double2* __restrict__ tw) {
+                                           const double2* __restrict__ tw, unsigned sw = 0) {
   stockham_pass<PL, 0, DIR>(v, t, tw);
   if constexpr (PL::P > 1) {
-    exchange<PL, 0, C, COLFAST>(v, t, c, lds);
+    exchange<PL, 0, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 1, DIR>(v, t, tw);
   }
   if constexpr (PL::P > 2) {
-    exchange<PL, 1, C, COLFAST>(v, t, c, lds);
+    exchange<PL, 1, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 2, DIR>(v, t, tw);
   }
   if constexpr (PL::P > 3) {
-    exchange<PL, 2, C, COLFAST>(v, t, c, lds);
+    exchange<PL, 2, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 3, DIR>(v, t, tw);
   }
 }

@@ -116,6 +116,15 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   const int c = q % C2;
   const int t = q / C2;
   double* ldsk = lds + k * L::SIZE;
+  // Lanes 2i and 2i+1 address the same slot of their component's half of LDS, and the
+  // halves are whole 64-bank lines apart, so every exchange access would be a 2-way bank
+  // conflict.  Component 2 XORs its slot index with 28 instead: within a 32-slot line
+  // that moves it to another column group (bit 2) and row slot (bits 3-4), which takes
+  // the partner lane to a bank of its own for every plan's exchange pattern, for groups
+  // of 16 and of 32 lanes alike.  Measured at 192^3: 608 -> 0 conflict cycles per wave,
+  // the same VALU count.
+  static_assert(L::SIZE % 32 == 0, "the slot XOR stays inside whole 32-slot lines");
+  const unsigned swz = k ? 28u : 0u;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
   if constexpr (SEQ == SP_ADJ) {
     for (int i = tid; i < N; i += 256) {
@@ -156,7 +165,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
 #pragma unroll
       for (int m = 0; m < E; ++m) opre[m] = p.cpl[base + (unsigned)m * step];
     }
-    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     if (p.ust1 && active) {   // keep the pre-step state for the adjoint's segment recomputation
       double2* __restrict__ us = k ? p.ust2 : p.ust1;
 #pragma unroll
@@ -209,7 +218,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       const double2 r = csub(cmul(cosom, v[m]), cmul(soff, o));   // solve_split.m:90-91
       v[m] = mul_cexp(r, -a * p.dr, -a * p.di, has_re);
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     if (active) {
 #pragma unroll
       for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
@@ -225,7 +234,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
     //                    A_k = h (g_1k q_1 + g_2k q_2)
     //   gradient         dL/dom0 += -h (s Re(conj(m_k) s_k) + c Re(conj(m_k) z))
     //                    dL/dkw  += sigma_k s X Im(conj(m_k) z),  sigma = +1, -1
-    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     const double2* __restrict__ usk = k ? p.ust2 : p.ust1;
     const unsigned base2 = opaque(base);
     const double mu = p.st->mu;
@@ -262,7 +271,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       g0 -= h * (sn * re1 + cs * re2);
       gk += sig * sn * X * im2;
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     const unsigned base3 = opaque(base);
     if (active) {
 #pragma unroll
@@ -292,13 +301,13 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       const double a = (Vk[idx] + gk1 * n1) + gk2 * n2;         // 49-54
       if (active) pk[idx] = exp_cw(-a * p.dr);                 // exp(-tmp2 dt) (70-71)
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     if (active) {
 #pragma unroll
       for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ITPB) {
-    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     const double* __restrict__ pk = p.pbuf + (k ? p.npts : 0u);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -323,13 +332,13 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
         v[m] = cscale(csub(cmul(cosom, v[m]), cmul(soff, o)), f);   // GPEtaskSpinor 70-71
       }
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     if (active) {
 #pragma unroll
       for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ITPC) {
-    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     double acc = 0.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) acc += cabs2(v[m]);
@@ -366,7 +375,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       const double2 ow = cmul(om, o);
       accX += own1 * (v[m].x * ow.x + v[m].y * ow.y);   // Re conj(psi1) Omega psi2
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl);
+    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const double e = p.keax[t + T * m] * cabs2(v[m]);
