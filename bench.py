@@ -247,6 +247,11 @@ def main():
                 "frac": round(dom["alg_GBps"] / PEAK_HBM_GBPS, 4), "traffic": traffic, "kernel": dom["kernel"],
                 "alg_bytes_per_launch": dom["alg_bytes"],
                 "traffic_source": f"{tsrc} (FETCH_SIZE/WRITE_SIZE, copy-calibrated, committed)" if traffic else None}
+    if dom["kernel"] == "ymid_x":
+        # what the counters show for this sweep (profiles/sweep256_c2_counters.md)
+        roofline["limiter"] = ("L2 request rate: each lane moves 16-byte pieces of lines shared by 8 tiles "
+                               "(8x the read requests of a line-coalesced sweep, TA busy 71 %); the two-sweep "
+                               "step it belongs to is faster than the four-sweep step at 0.77 of peak")
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
 
     # ------------------------------------------------- host-inclusive drop-in call
