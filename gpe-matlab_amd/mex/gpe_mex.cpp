@@ -35,6 +35,7 @@ struct Task {
   gpe_handle* h = nullptr;
   int ndims = 2;
   int n[3] = {1, 1, 1};   // nx, ny, nz
+  int ncomp = 0;          // components of the last accepted 'params' (0: none yet)
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
 
@@ -220,6 +221,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     p.n_recalc = as_int(prhs[7]);
     p.nlincpl = mxGetScalar(prhs[8]);
     check(gpe_set_params(t.h, &p), t.h, "gpe_set_params");
+    t.ncomp = p.ncomp;
   } else if (cmd == "omega") {
     // the omega property (GPEtask.m:7) as set_kinop.m:5-7 reads it
     need(nrhs, 3, c);
@@ -296,16 +298,17 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     set_out(nlhs, plhs, 0, column(m, 5));
   } else if (cmd == "applyham") {
     // applyham / applyh0 as arrays (GPEtask.m:73-95, GPEtaskSpinor.m:45-86):
-    // H = gpe_mex('applyham', h, ncomp, phi, h0_only); phi = [] takes the current state.
-    // Two components: phi and H are [grid.size, 2], the component along the last dimension.
-    need(nrhs, 5, c);
+    // H = gpe_mex('applyham', h, phi, h0_only); phi = [] takes the current state.
+    // Two components (a 2x2 g in 'params'): phi and H are [grid.size, 2], the component
+    // along the last dimension.  The output is sized from the task's component count.
+    need(nrhs, 4, c);
     Task& t = task_of(prhs[1]);
-    const int nc = as_int(prhs[2]);
-    if (nc != 1 && nc != 2) mexErrMsgIdAndTxt("gpe:args", "applyham: ncomp must be 1 or 2");
+    const int nc = t.ncomp;
+    if (nc != 1 && nc != 2) mexErrMsgIdAndTxt("gpe:args", "applyham: call 'params' first");
     std::vector<double> tmp;
-    const double* in = mxIsEmpty(prhs[3]) ? nullptr : complexes(prhs[3], (size_t)nc * t.npts(), tmp, "phi");
+    const double* in = mxIsEmpty(prhs[2]) ? nullptr : complexes(prhs[2], (size_t)nc * t.npts(), tmp, "phi");
     mxArray* out = grid_complex(t, nc);
-    check(gpe_applyham(t.h, in, cdata(out), as_int(prhs[4])), t.h, "gpe_applyham");
+    check(gpe_applyham(t.h, in, cdata(out), as_int(prhs[3])), t.h, "gpe_applyham");
     set_out(nlhs, plhs, 0, out);
   } else if (cmd == "itp") {
     // groundstate_itp.m:20-112 after process_init_state:
