@@ -40,7 +40,7 @@ PEAK_HBM_GBPS = 8000.0
 SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40, "kex": 24, "ke": 16,
              "itpa": 40, "itpb": 48, "closem": 40, "rkin": 32, "ropen": 40, "rmid": 40, "rclosem": 40,
              "ritpa": 40, "ritpb": 48, "sopen": 40, "smid": 40, "kinm": 32,
-             "ykin": 32, "yopen": 40, "ymid": 40, "yclosem": 40}
+             "ykin": 32, "yopen": 40, "ymid": 40, "yclosem": 40, "ysopen": 40, "ysmid": 40, "ykinm": 32}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 STEP_BYTES_3D_YSPLIT = 72  # two-sweep step: 2 x 32 B + 8 B of V
 
@@ -784,11 +784,14 @@ def bench_sgpe(args):
     steps = nout * 50
     rate = steps / el
     gbps = STEP_BYTES_3D * args.n ** 3 * rate / 1e9
+    two = args.n == 256 and os.environ.get("GPE_YSPLIT") != "0"
     log(f"sgpe: {steps} steps in {el:.3f}s -> {rate:.1f} steps/s, {gbps:.0f} GB/s algorithmic")
     _line("stochastic GPE split-steps/s (SGPEtask.solve_split 3D 256^3)", round(rate, 2), "split-steps/s", steps, 50,
           el, True, {"workload": "SGPEtask 3D grid3d(12,256,...) V=0.5(X^2+Y^2+16Z^2), g=0.02, gamma=0.01, T=20, "
                                  "ecut=60, mu_init=75, dt=0.002, n_recalc=10", "grid": [args.n] * 3},
           step_alg_GBps=round(gbps, 1), step_roofline_frac=round(gbps / PEAK_HBM_GBPS, 4),
+          schedule=("two sweeps per step, y transform split 16 x 16 (step_alg_GBps counts the 136 B/pt "
+                    "four-sweep schedule)") if two else "four sweeps per step (136 B/pt)",
           parity_rel_l2_vs_oracle_10_steps=parity)
 
 

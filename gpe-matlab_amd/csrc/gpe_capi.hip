@@ -176,6 +176,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->kax_c);
   for (double* b : h->Vmu) (void)hipFree(b);
   (void)hipFree(h->sg_kmax);
+  (void)hipFree(h->sg_kmaxB);
   (void)hipFree(h->vtd_G);
   (void)hipFree(h->dc_buf);
   (void)hipFree(h->vtd_V0);

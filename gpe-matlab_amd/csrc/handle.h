@@ -92,6 +92,7 @@ struct gpe_handle {
   double sg_var = 0;
   unsigned long long sg_seed = 0;
   int* sg_kmax = nullptr;         // per mid-axis column: largest |k| index with kk < ecut
+  int* sg_kmaxB = nullptr;        // the same in the y-split z sweep's column order
   // two-component block-end reductions: potential at task.current_time (gpe_set_potential_mu)
   double* Vmu[2] = {nullptr, nullptr};
   bool use_vmu = false;
@@ -391,7 +392,16 @@ inline int build_step_tables(gpe_handle* h, std::complex<double> dt) {
   return build_kinetic(h, dt, h->kcol, h->kax, 1.0, 0.0, ysplit_eligible(h) ? h->kcolB : nullptr);
 }
 
-// Buffers of the y-split step (allocated once) and the potential in tile order.
+// The permuted potential of the y-split x sweeps, refreshed when V[0] was written.
+inline int ysplit_refresh_v(gpe_handle* h) {
+  if (h->vp_gen != h->v_gen) {
+    HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz, h->stream));
+    h->vp_gen = h->v_gen;
+  }
+  return GPE_OK;
+}
+
+// Buffers of the y-split step (allocated once) and the potential in row order.
 inline int ysplit_prepare(gpe_handle* h) {
   if (!ysplit_eligible(h)) return GPE_OK;
   if (!h->twy) {
@@ -401,12 +411,7 @@ inline int ysplit_prepare(gpe_handle* h) {
   }
   if (!h->kcolB) HIPCHK(h, hipMalloc(&h->kcolB, (size_t)h->nx * h->ny * sizeof(double2)));
   if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
-  if (h->vp_gen != h->v_gen) {
-    HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           (unsigned)GPE_YSPLIT_NA, h->stream));
-    h->vp_gen = h->v_gen;
-  }
-  return GPE_OK;
+  return ysplit_refresh_v(h);
 }
 
 inline int read_state(gpe_handle* h, DevState* out) {
@@ -471,35 +476,40 @@ inline SweepParams ysplit_params(gpe_handle* h) {
   return p;
 }
 
+// z-sweep parameters of the y-split step (kinetic factor in the split's order).
+inline SweepParams ysplit_zparams(gpe_handle* h) {
+  SweepParams p = base_params(h, 2, h->psi[0], h->V[0]);
+  p.kcol = h->kcolB;
+  p.twy = h->twy;
+  return p;
+}
+
+// One y-split x sweep, timed like the other sweeps when profiling.
+inline int ysplit_xsweep(gpe_handle* h, int seq, const SweepParams& px, int* nb) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof) {
+    e0 = prof_event(h);
+    e1 = prof_event(h);
+    if (e0) hipEventRecord(e0, h->stream);
+  }
+  hipError_t e = gpe::launch_ysplit_x(h->nx, seq, px, h->stream, nb);
+  if (h->prof && e0 && e1) {
+    hipEventRecord(e1, h->stream);
+    h->prof_recs.push_back({seq * 3 + 0, {e0, e1}});
+  }
+  if (e != hipSuccess) return fail(h, GPE_EHIP, "y-split x sweep (seq %d): %s", seq, hipGetErrorString(e));
+  return GPE_OK;
+}
+
 // One block of solve_split.m:53-78 as two sweeps per step (y factor over b with the x
 // sweep, y factor over a with the z sweep): same operations as run_block, the y
 // transform factored 16 x 16.
 inline int run_block_ysplit(gpe_handle* h, double time2) {
   int rc;
-  if (h->vp_gen != h->v_gen) {
-    HIPCHK(h, gpe::launch_ysplit_permute_v(h->V[0], h->Vp, (unsigned)h->nx, (unsigned)h->nz,
-                                           (unsigned)GPE_YSPLIT_NA, h->stream));
-    h->vp_gen = h->v_gen;
-  }
-  SweepParams px = ysplit_params(h);
-  SweepParams pz = base_params(h, 2, h->psi[0], h->V[0]);
-  pz.kcol = h->kcolB;
-  pz.twy = h->twy;
-  auto xs = [&](int seq, int* nb) -> int {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (h->prof) {
-      e0 = prof_event(h);
-      e1 = prof_event(h);
-      if (e0) hipEventRecord(e0, h->stream);
-    }
-    hipError_t e = gpe::launch_ysplit_x(h->nx, seq, px, h->stream, nb);
-    if (h->prof && e0 && e1) {
-      hipEventRecord(e1, h->stream);
-      h->prof_recs.push_back({seq * 3 + 0, {e0, e1}});
-    }
-    if (e != hipSuccess) return fail(h, GPE_EHIP, "y-split x sweep (seq %d): %s", seq, hipGetErrorString(e));
-    return GPE_OK;
-  };
+  if ((rc = ysplit_refresh_v(h))) return rc;
+  const SweepParams px = ysplit_params(h);
+  const SweepParams pz = ysplit_zparams(h);
+  auto xs = [&](int seq, int* nb) { return ysplit_xsweep(h, seq, px, nb); };
   for (int i = 0; i < h->n_rec; ++i) {
     if ((rc = xs(i > 0 ? gpe::S_YMID : gpe::S_YOPEN, nullptr))) return rc;
     if ((rc = sweep(h, 2, gpe::S_YKIN, pz))) return rc;

@@ -6,7 +6,9 @@
 // closing half phase (72) and ncur = integrate(|psi|^2) (73-75).  mu = mu_init is
 // fixed and the state is never renormalised.  The noise of global step s at grid
 // point p is the complex normal variate of Philox counter (p, s) under the task's
-// noise seed (philox_dev.h), so a run is reproducible and resumable.
+// noise seed (philox_dev.h), so a run is reproducible and resumable.  On a 3D grid with
+// a 256-point y axis a step is the two sweeps of the y-split step (S_YSOPEN / S_YSMID
+// with the noise, S_YKIN / S_YKINM), as for solve_split; elsewhere the four sweeps.
 #include <cmath>
 #include <vector>
 
@@ -24,8 +26,42 @@ SweepParams sg_params(gpe_handle* h, int axis) {
   return p;
 }
 
+// The block of sg_block on the y-split step.
+int sg_block_ysplit(gpe_handle* h, long long step0, int* nb) {
+  int rc;
+  if ((rc = ysplit_refresh_v(h))) return rc;
+  const bool noisy = h->sg_var != 0.0;
+  SweepParams px = ysplit_params(h);
+  px.noise_var = h->sg_var;
+  px.noise_seed = h->sg_seed;
+  SweepParams pz = ysplit_zparams(h);
+  pz.kmax = h->sg_kmaxB;
+  const int kin = h->sg_mask ? gpe::S_YKINM : gpe::S_YKIN;
+  for (int i = 0; i < h->n_rec; ++i) {
+    px.noise_step = (unsigned long long)(step0 + i);
+    const int xs = i == 0 ? (noisy ? gpe::S_YSOPEN : gpe::S_YOPEN) : (noisy ? gpe::S_YSMID : gpe::S_YMID);
+    if ((rc = ysplit_xsweep(h, xs, px, nullptr))) return rc;
+    if ((rc = sweep(h, 2, kin, pz))) return rc;
+  }
+  return ysplit_xsweep(h, gpe::S_YCLOSEM, px, nb);   // 72-75 (only the norm is used)
+}
+
 int sg_block(gpe_handle* h, long long step0, double* ncur) {
   int rc;
+  if (ysplit_active(h)) {
+    int nb = 0;
+    if ((rc = sg_block_ysplit(h, step0, &nb))) return rc;
+    FinParams f = fin_params(h, gpe::FIN_SUMS);
+    f.cnt[gpe::SL_A] = nb;
+    HIPCHK(h, gpe::launch_finalize(f, h->stream));
+    if (ncur) {
+      DevState s;
+      if ((rc = read_state(h, &s))) return rc;
+      if (!std::isfinite(s.sA)) return fail(h, GPE_ENONFINITE, "SGPE block norm is not finite (%g)", s.sA);
+      *ncur = s.sA * h->w;
+    }
+    return GPE_OK;
+  }
   const bool noisy = h->sg_var != 0.0;
   const int kin = h->sg_mask ? gpe::S_KINM : gpe::S_KIN;
   for (int i = 0; i < h->n_rec; ++i) {
@@ -74,7 +110,9 @@ int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a) {
   const std::complex<double> dt = cdiv({0.0, a->ddt}, {1.0, h->prm.gamma});   // 29
   h->dr = dt.real();
   h->di = dt.imag();
-  if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;                 // 41 / 43
+  if ((rc = ysplit_prepare(h))) return rc;
+  if ((rc = build_kinetic(h, dt, h->kcol, h->kax, 1.0, 0.0, ysplit_eligible(h) ? h->kcolB : nullptr)))
+    return rc;                                                                   // 41 / 43
   h->sg_mask = a->ecut > 0 && a->T > 0;                                         // 39
   h->sg_var = std::sqrt(a->T * h->prm.gamma * a->ddt / h->w);                 // 47
   h->sg_seed = a->seed;
@@ -102,6 +140,19 @@ int gpe_sgpe_begin(gpe_handle* h, const gpe_sgpe_args* a) {
     }
     if (!h->sg_kmax) HIPCHK(h, hipMalloc(&h->sg_kmax, kmax.size() * sizeof(int)));
     HIPCHK(h, hipMemcpy(h->sg_kmax, kmax.data(), kmax.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (ysplit_eligible(h)) {
+      // the y-split z sweep's column order (the kcolB order of build_kinetic): entry
+      // pos + 256 ix belongs to ky = bitrev4(pos / 16) + 16 (pos mod 16)
+      std::vector<int> kb(kmax.size());
+      for (int ix = 0; ix < h->nx; ++ix)
+        for (int pos = 0; pos < 256; ++pos) {
+          const int r = pos >> 4;
+          const int ky = (((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3)) + 16 * (pos & 15);
+          kb[(size_t)pos + 256 * (size_t)ix] = kmax[(size_t)ky + 256 * (size_t)ix];
+        }
+      if (!h->sg_kmaxB) HIPCHK(h, hipMalloc(&h->sg_kmaxB, kb.size() * sizeof(int)));
+      HIPCHK(h, hipMemcpy(h->sg_kmaxB, kb.data(), kb.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
   }
   DevState s;
   if ((rc = read_state(h, &s))) return rc;

@@ -212,23 +212,20 @@ hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, 
   }
 }
 
-// Potential in the row order of the y-split x sweeps (tiles of na values of a):
-// Vp[(((z tpp + ap) 16 + b) nx + x) na + g] = V(y = na ap + g + 16 b, x, z), one thread
-// per output element.
+// Potential in the row order of the y-split x sweeps:
+// Vp[((z 16 + a) 16 + b) nx + x] = V(y = a + 16 b, x, z), one thread per output element.
 __global__ void __launch_bounds__(256) ysplit_permute_v_kernel(const double* __restrict__ V, double* __restrict__ Vp,
-                                                               unsigned nx, unsigned na, unsigned total) {
+                                                               unsigned nx, unsigned total) {
   const unsigned i = blockIdx.x * 256u + threadIdx.x;
   if (i >= total) return;
-  const unsigned tpp = 16u / na;
-  const unsigned g = i % na, x = (i / na) % nx, b = (i / (na * nx)) % 16u, tile = i / (16u * na * nx);
-  const unsigned ap = tile % tpp, z = tile / tpp;
-  const unsigned y = na * ap + g + 16u * b;
-  Vp[i] = V[y + 256u * x + 256u * nx * z];
+  const unsigned x = i % nx, b = (i / nx) % 16u, tile = i / (16u * nx);
+  const unsigned a = tile % 16u, z = tile / 16u;
+  Vp[i] = V[a + 16u * b + 256u * x + 256u * nx * z];
 }
 
-hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na, hipStream_t s) {
+hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, hipStream_t s) {
   const unsigned total = 256u * nx * nz;
-  hipLaunchKernelGGL(ysplit_permute_v_kernel, dim3((total + 255u) / 256u), dim3(256), 0, s, V, Vp, nx, na, total);
+  hipLaunchKernelGGL(ysplit_permute_v_kernel, dim3((total + 255u) / 256u), dim3(256), 0, s, V, Vp, nx, total);
   return hipGetLastError();
 }
 

@@ -37,13 +37,17 @@ enum Seq : int {
   S_SMID = 18,    // inverse, full phase, + noise, forward
   S_KINM = 19,    // S_KIN with the energy cut-off mask kk < ecut
   // 3D steps with a 256-point y axis split as 16 x 16 (y = a + 16 b): the b factor runs
-  // in the x sweeps (tiles of 2 a x 16 b x nx), the a factor in the z sweep, so a step
+  // in the x sweeps (tiles of one a x 16 b x nx), the a factor in the z sweep, so a step
   // is two sweeps; between them the state is x-spectral with the y factor over b applied
   S_YKIN = 20,    // y factor over a (16 adjacent pencils), forward, x kcol[col]*kax[i], inverse, y factor back
   S_YOPEN = 21,   // scale, block-open half phase, forward x, y factor over b
   S_YMID = 22,    // y factor over b back, inverse x, full phase, forward x, y factor over b
   S_YCLOSEM = 23, // y factor over b back, inverse x, full and closing half phase; block-end sums
-  S_NSEQ = 24
+  // the stochastic GPE on the same split: noise after the phase, and the cut-off mask
+  S_YSOPEN = 24,  // S_YOPEN, + noise before the forward x transform
+  S_YSMID = 25,   // S_YMID, + noise before the forward x transform
+  S_YKINM = 26,   // S_YKIN with the energy cut-off mask kk < ecut
+  S_NSEQ = 27
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -155,18 +159,13 @@ hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipS
                         int var = 0);
 bool sweep_length_supported(int n);
 hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks);
-// x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM) for x length n; p.V is the
-// potential in tile order (launch_ysplit_permute_v)
-// na (1 or 2) values of a per tile; pv 0: the phase sweeps' plan, 1: Plan<n>.
+// x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM, S_YSOPEN, S_YSMID) for x
+// length n; p.V is the potential in row order (launch_ysplit_permute_v)
 hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, int* nblocks);
 bool ysplit_x_supported(int n);
-// Vp[(z*(16/na) + ap)*16 na*nx + x*16 na + c] = V[(na ap + c/16 + 16 (c%16)) + 256 x + 256 nx z]
-// values of a per x-sweep tile of the y-split step (row order of the permuted potential)
-#ifndef GPE_YSPLIT_NA
-#define GPE_YSPLIT_NA 1
-#endif
-hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, unsigned na,
-                                   hipStream_t s);
+// Vp[((z 16 + a) 16 + b) nx + x] = V[a + 16 b + 256 x + 256 nx z]: the potential in the
+// row order of the y-split x sweeps
+hipError_t launch_ysplit_permute_v(const double* V, double* Vp, unsigned nx, unsigned nz, hipStream_t s);
 hipError_t launch_rowdft_selftest(const double2* in, double2* out_f, double2* out_i, hipStream_t s);
 
 }  // namespace gpe

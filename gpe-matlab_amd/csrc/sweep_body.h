@@ -85,7 +85,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const int c = COLFAST ? tid % C : tid / T;
   const int t = COLFAST ? tid / C : tid % T;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
-  if constexpr (SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN) {
+  if constexpr (SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM) {
     for (int i = tid; i < N; i += 256) twl[N + i] = p.kax[i];
     __syncthreads();  // single-pass plans have no exchange barrier before the first use
   }
@@ -153,7 +153,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
-  } else if constexpr (SEQ == S_YKIN) {
+  } else if constexpr (SEQ == S_YKIN || SEQ == S_YKINM) {
     // Every 16 consecutive pencils of the tile are one y block (a = 0..15 of
     // y = a + 16 s).  z transform in the pencil layout, then the tile moves to the row
     // layout (thread q: kz index q / G of the 16 pencils of block g = q % G), where
@@ -168,8 +168,21 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
     DFT<16, -1>::run(v);
     const double2 kz = twl[N + tid / G];
     const double2* __restrict__ kc = p.kcol + tile * (unsigned)C + 16u * (unsigned)(tid % G);
+    if constexpr (SEQ == S_YKINM) {
+      // the cut-off mask of S_KINM (SGPEtask/solve_split.m:39-41): kmax in the same
+      // split order as kcol, kz index tid / G
+      const int* __restrict__ km = p.kmax + tile * (unsigned)C + 16u * (unsigned)(tid % G);
+      const int j = tid / G, aj = j <= N / 2 ? j : N - j;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc[m], kz), v[m]);
+      for (int m = 0; m < E; ++m) {
+        const double keep = aj <= km[m] ? 1.0 : 0.0;
+        const double2 f = cmul(kc[m], kz);
+        v[m] = cmul(make_double2(f.x * keep, f.y * keep), v[m]);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc[m], kz), v[m]);
+    }
     DFT<16, +1>::run(v);
     row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
@@ -339,7 +352,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
 template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
 __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN ? 2 * N : N];
+  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM ? 2 * N : N];
   __shared__ double red[4];
   unsigned bid = blockIdx.x, nbk = gridDim.x;
   double2* psi = p.psi;

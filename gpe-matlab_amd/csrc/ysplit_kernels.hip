@@ -1,20 +1,19 @@
 // x sweeps of the y-split 3D step for one x length (-DGPE_N=<nx>): S_YOPEN, S_YMID,
-// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65).
+// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65) and the stochastic S_YSOPEN, S_YSMID
+// (SGPEtask/solve_split.m:55-59, 69 then 59).
 //
 // The y axis has 256 points, y = a + 16 b.  A work-group takes the tile of one z
-// plane with NA values of a (a = NA ap + g) and every b: C = 16 NA x-pencils, pencil
-// c <-> (a = NA ap + c / 16, b slot r = c % 16), T = nx/16 threads each, 16 elements
-// per thread.  The x transforms run in that pencil layout (tid = t C + c); the tile
-// then moves through LDS to the row layout (thread q = NA x + g holds the 16 b slots
-// of one (a, x), row_exchange16), where the y factor over b is a 16-point DFT in
-// registers and the phase step applies.  Between the sweeps the b slot r holds the
-// spectral index k1 = bitrev4(r), times the twiddle W256^{a k1} of the 16 x 16 split;
-// the z sweep applies the factor over a.
+// plane and one value of a with every b: C = 16 x-pencils, pencil c <-> b slot c,
+// T = nx/16 threads each, 16 elements per thread.  The x transforms run in that pencil
+// layout (tid = t C + c); the tile then moves through LDS to the row layout (thread
+// q = x holds the 16 b slots of one (a, x), row_exchange16), where the y factor over b
+// is a 16-point DFT in registers and the phase step applies.  Between the sweeps the b
+// slot r holds the spectral index k1 = bitrev4(r), times the twiddle W256^{a k1} of the
+// 16 x 16 split; the z sweep applies the factor over a.
 //
-// The potential is read in row order (launch_ysplit_permute_v): for each b, the NA nx
-// values of the tile's (x, g) rows are contiguous.
-#include <type_traits>
-
+// The potential is read in row order (launch_ysplit_permute_v): for each b, the nx
+// values of the tile's rows are contiguous.
+#include "philox_dev.h"
 #include "rowdft_dev.h"
 #include "sweep_body.h"
 
@@ -43,14 +42,15 @@ __device__ __forceinline__ double block_sum_w(double v, double* red) {
 
 constexpr int br4c(int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); }
 
-template <int N, int NA>
-constexpr int ythreads() { return 16 * NA * (N / 16); }
+template <int N>
+constexpr int ythreads() { return 16 * (N / 16); }
 
-template <int N, int SEQ, int NA>
-__global__ void __launch_bounds__((ythreads<N, NA>())) __attribute__((amdgpu_waves_per_eu(2)))
+template <int N, int SEQ>
+__global__ void __launch_bounds__((ythreads<N>())) __attribute__((amdgpu_waves_per_eu(2)))
 ysplit_x_kernel(const SweepParams p) {
   using PL = Plan<N>;
-  constexpr int T = PL::T, E = PL::E, C = 16 * NA, NT = C * T, NW = NT / 64, TPP = 16 / NA;
+  constexpr int T = PL::T, E = PL::E, C = 16, NT = C * T, NW = NT / 64;
+  constexpr bool OPEN = SEQ == S_YOPEN || SEQ == S_YSOPEN, NOISE = SEQ == S_YSOPEN || SEQ == S_YSMID;
   static_assert(E == 16 && NT % 64 == 0, "16 elements per thread, whole wavefronts");
   __shared__ double lds[Lds<N, C, true>::SIZE];
   __shared__ double2 twl[N];
@@ -61,17 +61,16 @@ ysplit_x_kernel(const SweepParams p) {
   for (int i = tid; i < 256; i += NT) twy[i] = p.twy[i];
   __syncthreads();
 
-  // the TPP tiles of a z plane go to one XCD (group b runs on XCD b % 8); gridDim = TPP nz
+  // the 16 tiles of a z plane go to one XCD (group b runs on XCD b % 8); gridDim = 16 nz
   const unsigned bi = blockIdx.x, xcd = bi & 7u, s8 = bi >> 3;
-  const unsigned ap = s8 % (unsigned)TPP, z = (s8 / (unsigned)TPP) * 8u + xcd;
+  const unsigned ar = s8 % 16u, z = (s8 / 16u) * 8u + xcd;
   const unsigned ny = p.gny, nx = p.gnx;
   // pencil layout
   const int c = tid % C, t = tid / C;
-  const unsigned base = (unsigned)(NA * (int)ap + (c >> 4) + 16 * (c & 15)) + ny * (unsigned)t + ny * nx * z;
+  const unsigned base = ar + 16u * (unsigned)c + ny * (unsigned)t + ny * nx * z;
   const unsigned step = (unsigned)T * ny;
-  // row layout: a = NA ap + g of row q = tid
-  const int ar = NA * (int)ap + tid % NA;
-  const unsigned vrow = (z * (unsigned)TPP + ap) * 16u * (unsigned)NT + (unsigned)tid;
+  // row layout: row q = tid is x = tid of the tile's (a, z)
+  const unsigned vrow = (z * 16u + ar) * 16u * (unsigned)NT + (unsigned)tid;
 
   double2 v[E];
 #pragma unroll
@@ -86,7 +85,7 @@ ysplit_x_kernel(const SweepParams p) {
   const double mu = p.st->mu;
   double2 u[16];   // row layout, b natural (real space) or k1 natural (spectral)
 
-  if constexpr (SEQ == S_YOPEN) {
+  if constexpr (OPEN) {
     // real-space state: slot r holds b = r.  solve_split.m:55-56 with the pending
     // renormalisation (solve_split.m:72-73)
     row_exchange16<N, C, true>(v, lds);
@@ -102,7 +101,7 @@ ysplit_x_kernel(const SweepParams p) {
     row_exchange16<N, C, true>(v, lds);
     // undo the split twiddle and the y factor over b left by the previous x sweep
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) u[k1] = cmulc(v[br4c(k1)], twy[ar * k1]);
+    for (int k1 = 0; k1 < 16; ++k1) u[k1] = cmulc(v[br4c(k1)], twy[ar * (unsigned)k1]);
     DFT<16, +1>::run(u);
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
@@ -116,10 +115,21 @@ ysplit_x_kernel(const SweepParams p) {
       u[b] = x;
     }
   }
-  if constexpr (SEQ == S_YOPEN || SEQ == S_YMID) {
+  if constexpr (NOISE) {
+    // SGPEtask/solve_split.m:59: the noise of this step, one complex normal per point,
+    // Philox counter (natural grid index of the point, global step)
+    const unsigned pt0 = ar + ny * (unsigned)tid + ny * nx * z;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const double2 w = complex_normal(pt0 + 16u * (unsigned)b, p.noise_step, p.noise_seed);
+      u[b].x = u[b].x + p.noise_var * w.x;
+      u[b].y = u[b].y + p.noise_var * w.y;
+    }
+  }
+  if constexpr (SEQ != S_YCLOSEM) {
     DFT<16, -1>::run(u);
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * k1]);
+    for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * (unsigned)k1]);
     row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
 #pragma unroll
@@ -159,25 +169,20 @@ ysplit_x_kernel(const SweepParams p) {
 #define GPE_CAT2(a, b) a##b
 #define GPE_CAT(a, b) GPE_CAT2(a, b)
 
-// Templated on the length as well: this file is compiled once per length, and a helper
-// named only by NA would be one merged symbol across the objects.
-template <int N, int NA>
-static hipError_t launch_na(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
-  const dim3 grid((16u / NA) * p.gnz), block(ythreads<N, NA>());
+hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
+  constexpr int N = GPE_N;
+  if (p.gny != 256u || p.gnx != (unsigned)N || p.gnz % 8u != 0u || p.gnz == 0u) return hipErrorInvalidValue;
+  const dim3 grid(16u * p.gnz), block(ythreads<N>());
   switch (seq) {
-    case S_YOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YOPEN, NA>), grid, block, 0, s, p); break;
-    case S_YMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YMID, NA>), grid, block, 0, s, p); break;
-    case S_YCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEM, NA>), grid, block, 0, s, p); break;
+    case S_YOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YOPEN>), grid, block, 0, s, p); break;
+    case S_YMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YMID>), grid, block, 0, s, p); break;
+    case S_YCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEM>), grid, block, 0, s, p); break;
+    case S_YSOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YSOPEN>), grid, block, 0, s, p); break;
+    case S_YSMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YSMID>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   if (nblocks) *nblocks = (int)grid.x;
   return hipGetLastError();
-}
-
-hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipStream_t s, int* nblocks) {
-  constexpr int N = GPE_N;
-  if (p.gny != 256u || p.gnx != (unsigned)N || p.gnz % 8u != 0u || p.gnz == 0u) return hipErrorInvalidValue;
-  return launch_na<N, GPE_YSPLIT_NA>(seq, p, s, nblocks);
 }
 
 }  // namespace gpe

@@ -64,3 +64,33 @@ def test_sgpe_callback_term_stops():
     ts.user_callback = lambda task: "TERM" if task.current_iter >= 2 else ""
     ts.solve_split(0.005, 20, 5)
     assert ts.current_iter == 2
+
+
+def _with_schedule(value, fn):
+    import os
+    old = os.environ.get("GPE_YSPLIT")
+    os.environ["GPE_YSPLIT"] = value      # read when the task's device handle is created
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["GPE_YSPLIT"]
+        else:
+            os.environ["GPE_YSPLIT"] = old
+
+
+@pytest.mark.parametrize("ecut", [0.0, 6.0])
+def test_sgpe_two_sweep_step_on_a_256_point_y_axis(ecut):
+    """On a 3D grid with a 256-point y axis the stochastic step runs as two sweeps (noise
+    in the x sweep's row layout, the cut-off mask in the z sweep's split order); same
+    noise stream, so it matches the oracle and the four-sweep step of the library."""
+    n, xmax = (64, 256, 64), (6, 8, 5)
+    tg, to = _with_schedule("1", lambda: _pair(3, n, xmax, ecut=ecut))
+    pg = _with_schedule("1", lambda: tg.solve_split(0.005, 20, 2))
+    t4, _ = _with_schedule("0", lambda: _pair(3, n, xmax, ecut=ecut))
+    p4 = _with_schedule("0", lambda: t4.solve_split(0.005, 20, 2))
+    po = to.solve_split(0.005, 20, 2)
+    e_o, e_4 = rel_l2(pg, po), rel_l2(pg, p4)
+    print(f"SGPE two-sweep 64x256x64 ecut={ecut}: vs oracle {e_o:.2e}, vs four-sweep {e_4:.2e}", flush=True)
+    assert e_o < 1e-11 and e_4 < 1e-12
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
