@@ -332,6 +332,12 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
   return GPE_OK;
 }
 
+// The y-split 3D step can run on this handle (its tables are set up by ysplit_prepare).
+inline bool ysplit_eligible(const gpe_handle* h) {
+  return !h->ysplit_off && h->ndims == 3 && h->ny == 256 && h->prm.ncomp == 1 &&
+         gpe::ysplit_x_supported(h->nx) && gpe::ysplit_x_supported(h->nz) && h->nz % 8 == 0;
+}
+
 // Rotating-frame kinetic tables for complex dt (set_kinop.m:6-7):
 //   x half step  rtab_x[iy][kx] = exp(-(kx^2/2 - kx y omega) dt/2) / nx
 //   3D y/z step  kcol[iy + ny ix] = exp(-(ky^2/2 + ky x omega) dt), kax[kz] = exp(-kz^2/2 dt) / (ny nz)
@@ -367,6 +373,17 @@ inline int build_kinetic_rot(gpe_handle* h, std::complex<double> dt) {
     }
     HIPCHK(h, hipMemcpyAsync(h->kcol, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->kax, az.data(), az.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    if (h->kcolB && ysplit_eligible(h)) {
+      // the y-split z sweep's order of the same (ky, x) table (see build_kinetic)
+      std::vector<double2> tb(ty.size());
+      for (int ix = 0; ix < nx; ++ix)
+        for (int pos = 0; pos < 256; ++pos) {
+          const int r = pos >> 4;
+          const int ky = (((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3)) + 16 * (pos & 15);
+          tb[(size_t)pos + 256 * (size_t)ix] = ty[(size_t)ky + 256 * (size_t)ix];
+        }
+      HIPCHK(h, hipMemcpyAsync(h->kcolB, tb.data(), tb.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    }
   } else {
     if (!h->rtab_y) HIPCHK(h, hipMalloc(&h->rtab_y, (size_t)nx * ny * sizeof(double2)));
     for (int ix = 0; ix < nx; ++ix)
@@ -381,11 +398,6 @@ inline int build_kinetic_rot(gpe_handle* h, std::complex<double> dt) {
 }
 
 // The kinetic tables of the current frame.
-// The y-split 3D step can run on this handle (its tables are set up by ysplit_prepare).
-inline bool ysplit_eligible(const gpe_handle* h) {
-  return !h->ysplit_off && h->ndims == 3 && h->ny == 256 && h->omega == 0.0 && h->prm.ncomp == 1 &&
-         gpe::ysplit_x_supported(h->nx) && gpe::ysplit_x_supported(h->nz) && h->nz % 8 == 0;
-}
 
 inline int build_step_tables(gpe_handle* h, std::complex<double> dt) {
   if (h->omega != 0.0) return build_kinetic_rot(h, dt);
@@ -510,13 +522,15 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
   const SweepParams px = ysplit_params(h);
   const SweepParams pz = ysplit_zparams(h);
   auto xs = [&](int seq, int* nb) { return ysplit_xsweep(h, seq, px, nb); };
+  const bool rot = h->omega != 0.0;
+  const int s_open = rot ? gpe::S_YROPEN : gpe::S_YOPEN, s_mid = rot ? gpe::S_YRMID : gpe::S_YMID;
   for (int i = 0; i < h->n_rec; ++i) {
-    if ((rc = xs(i > 0 ? gpe::S_YMID : gpe::S_YOPEN, nullptr))) return rc;
+    if ((rc = xs(i > 0 ? s_mid : s_open, nullptr))) return rc;
     if ((rc = sweep(h, 2, gpe::S_YKIN, pz))) return rc;
   }
   FinParams f = fin_params(h, gpe::FIN_BLOCK);
   int nb = 0;
-  if ((rc = xs(gpe::S_YCLOSEM, &nb))) return rc;
+  if ((rc = xs(rot ? gpe::S_YRCLOSEM : gpe::S_YCLOSEM, &nb))) return rc;
   f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
   for (int a = 1; a < 3; ++a) {
     if ((rc = sweep(h, a, gpe::S_KE, base_params(h, a, h->psi[0], h->V[0]), &nb))) return rc;

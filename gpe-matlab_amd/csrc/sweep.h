@@ -47,7 +47,12 @@ enum Seq : int {
   S_YSOPEN = 24,  // S_YOPEN, + noise before the forward x transform
   S_YSMID = 25,   // S_YMID, + noise before the forward x transform
   S_YKINM = 26,   // S_YKIN with the energy cut-off mask kk < ecut
-  S_NSEQ = 27
+  // the rotating frame on the same split: x stays real between the sweeps, the x half
+  // steps Kx run around the phase in the x sweep, the z sweep takes the (ky, x) table
+  S_YROPEN = 27,   // scale, block-open half phase, Kx, y factor over b
+  S_YRMID = 28,    // y factor over b back, Kx, full phase, Kx, y factor over b
+  S_YRCLOSEM = 29, // y factor over b back, Kx, full and closing half phase; block-end sums
+  S_NSEQ = 30
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -159,8 +164,9 @@ hipError_t launch_sweep(int n, int seq, bool colfast, const SweepParams& p, hipS
                         int var = 0);
 bool sweep_length_supported(int n);
 hipError_t launch_spinor_sweep(int n, int seq, const SpinorParams& p, hipStream_t s, int* nblocks);
-// x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM, S_YSOPEN, S_YSMID) for x
-// length n; p.V is the potential in row order (launch_ysplit_permute_v)
+// x sweeps of the y-split 3D step (S_YOPEN, S_YMID, S_YCLOSEM, S_YSOPEN, S_YSMID, S_YROPEN,
+// S_YRMID, S_YRCLOSEM) for x length n; p.V is the potential in row order
+// (launch_ysplit_permute_v)
 hipError_t launch_ysplit_x(int n, int seq, const SweepParams& p, hipStream_t s, int* nblocks);
 bool ysplit_x_supported(int n);
 // Vp[((z 16 + a) 16 + b) nx + x] = V[a + 16 b + 256 x + 256 nx z]: the potential in the

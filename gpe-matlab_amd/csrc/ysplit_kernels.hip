@@ -1,6 +1,7 @@
 // x sweeps of the y-split 3D step for one x length (-DGPE_N=<nx>): S_YOPEN, S_YMID,
-// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65) and the stochastic S_YSOPEN, S_YSMID
-// (SGPEtask/solve_split.m:55-59, 69 then 59).
+// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65), the stochastic S_YSOPEN, S_YSMID
+// (SGPEtask/solve_split.m:55-59, 69 then 59) and the rotating-frame S_YROPEN, S_YRMID,
+// S_YRCLOSEM (the same with the x half steps of ssft_kin_step.m:22-25).
 //
 // The y axis has 256 points, y = a + 16 b.  A work-group takes the tile of one z
 // plane and one value of a with every b: C = 16 x-pencils, pencil c <-> b slot c,
@@ -51,6 +52,7 @@ ysplit_x_kernel(const SweepParams p) {
   using PL = Plan<N>;
   constexpr int T = PL::T, E = PL::E, C = 16, NT = C * T, NW = NT / 64;
   constexpr bool OPEN = SEQ == S_YOPEN || SEQ == S_YSOPEN, NOISE = SEQ == S_YSOPEN || SEQ == S_YSMID;
+  constexpr bool ROT = SEQ == S_YROPEN || SEQ == S_YRMID || SEQ == S_YRCLOSEM;
   static_assert(E == 16 && NT % 64 == 0, "16 elements per thread, whole wavefronts");
   __shared__ double lds[Lds<N, C, true>::SIZE];
   __shared__ double2 twl[N];
@@ -77,13 +79,101 @@ ysplit_x_kernel(const SweepParams p) {
   for (int m = 0; m < E; ++m) v[m] = p.psi[base + (unsigned)m * step];
   // the potential is loaded with the state, before the transform: at two waves per SIMD
   // nothing else would hide its latency at the phase
+  // (rotating frame: in the pencil layout, where its phase applies; the row order keeps
+  // each pencil's x values contiguous)
   double vv[16];
+  if constexpr (ROT) {
+    const unsigned vpen = ((z * 16u + ar) * 16u + (unsigned)c) * nx + (unsigned)t;
 #pragma unroll
-  for (int b = 0; b < 16; ++b) vv[b] = p.V[vrow + (unsigned)b * (unsigned)NT];
+    for (int m = 0; m < 16; ++m) vv[m] = p.V[vpen + (unsigned)(T * m)];
+  } else {
+#pragma unroll
+    for (int b = 0; b < 16; ++b) vv[b] = p.V[vrow + (unsigned)b * (unsigned)NT];
+  }
 
   const bool has_re = p.dr != 0.0;
   const double mu = p.st->mu;
   double2 u[16];   // row layout, b natural (real space) or k1 natural (spectral)
+
+  if constexpr (ROT) {
+    // Rotating frame (set_kinop.m:5-7, ssft_kin_step.m:22-25).  Between the sweeps the
+    // state is real in x and carries the y factor over b; the x half steps Kx =
+    // forward x, x rtab[y][kx] (incl. 1/nx), inverse x run in the pencil layout around
+    // the phase, and the y factor over b in the row layout before and after them.
+    const unsigned yrow = ar + 16u * (unsigned)c;
+    const double2* __restrict__ rrow = p.rtab + (size_t)yrow * N;
+    if constexpr (SEQ == S_YROPEN) {
+      const double sc = p.st->scale;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double2 x = cscale(v[m], sc);
+        const double aa = -(vv[m] - mu) - p.g * cabs2(x);
+        v[m] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);   // solve_split.m:55-56
+      }
+      table_kin<PL, C, true>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:23
+    } else {
+      row_exchange16<N, C, true>(v, lds);
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) u[k1] = cmulc(v[br4c(k1)], twy[ar * (unsigned)k1]);
+      DFT<16, +1>::run(u);
+#pragma unroll
+      for (int b = 0; b < 16; ++b) v[b] = u[b];
+      row_exchange16<N, C, false>(v, lds);
+      table_kin<PL, C, true>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:25
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double vm = vv[m] - mu;
+        const double aa = -vm - p.g * cabs2(v[m]);                        // solve_split.m:60
+        double2 x = mul_cexp(v[m], aa * p.dr, aa * p.di, has_re);          // solve_split.m:61
+        if constexpr (SEQ == S_YRCLOSEM) {
+          const double bb = vm + p.g * cabs2(x);                           // solve_split.m:63
+          x = mul_cexp(x, bb * p.dr * 0.5, bb * p.di * 0.5, has_re);       // solve_split.m:64
+        }
+        v[m] = x;
+      }
+      if constexpr (SEQ == S_YRMID) table_kin<PL, C, true>(v, t, c, lds, twl, rrow);   // next step's 23
+    }
+    if constexpr (SEQ != S_YRCLOSEM) {
+      row_exchange16<N, C, true>(v, lds);
+#pragma unroll
+      for (int b = 0; b < 16; ++b) u[b] = v[b];
+      DFT<16, -1>::run(u);
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * (unsigned)k1]);
+      row_exchange16<N, C, false>(v, lds);
+#pragma unroll
+      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+    } else {
+      // block-end sums of the stored real-space state (solve_split.m:65-77), the x
+      // kinetic term with the rotating-frame cross term of the pencil's y
+#pragma unroll
+      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+      double accA = 0.0, accB = 0.0, accC = 0.0, accK = 0.0;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double n = cabs2(v[m]);
+        accA += n;
+        accB += vv[m] * n;
+        accC += n * n;
+      }
+      fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
+      const double rc = p.rcoord[yrow];
+#pragma unroll
+      for (int m = 0; m < E; ++m) accK += ke_weight(p, t + T * m, rc) * cabs2(v[m]);
+      const double sa = block_sum_w<NW>(accA, red);
+      const double sb = block_sum_w<NW>(accB, red);
+      const double scc = block_sum_w<NW>(accC, red);
+      const double sk = block_sum_w<NW>(accK, red);
+      if (tid == 0) {
+        const unsigned wg = blockIdx.x;
+        p.partials[SL_A * p.maxwg + wg] = sa;
+        p.partials[SL_B * p.maxwg + wg] = sb;
+        p.partials[SL_C * p.maxwg + wg] = scc;
+        p.partials[p.slot_ke * p.maxwg + wg] = sk;
+      }
+    }
+    return;
+  }
 
   if constexpr (OPEN) {
     // real-space state: slot r holds b = r.  solve_split.m:55-56 with the pending
@@ -179,6 +269,9 @@ hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipSt
     case S_YCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEM>), grid, block, 0, s, p); break;
     case S_YSOPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YSOPEN>), grid, block, 0, s, p); break;
     case S_YSMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YSMID>), grid, block, 0, s, p); break;
+    case S_YROPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YROPEN>), grid, block, 0, s, p); break;
+    case S_YRMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YRMID>), grid, block, 0, s, p); break;
+    case S_YRCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YRCLOSEM>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   if (nblocks) *nblocks = (int)grid.x;

@@ -178,3 +178,35 @@ def test_ysplit_time_dependent_potential():
     err = rel_l2(pg, po)
     print(f"y-split with host Vtd: rel-L2 vs oracle {err:.3e}", flush=True)
     assert err < TOL
+
+
+@pytest.mark.parametrize("n,xmax", [((64, 256, 64), (6, 9, 5)), ((128, 256, 64), (7, 10, 5))])
+@pytest.mark.parametrize("gamma", [0.0, 0.02])
+def test_ysplit_rotating_frame_matches_oracle_and_four_sweeps(n, xmax, gamma):
+    """Scalar omega (set_kinop.m:5-7, ssft_kin_step.m:22-25): x stays real between the two
+    sweeps, the x half steps run around the phase in the x sweep and the z sweep takes
+    the (ky, x) table; against the oracle and the library's four-sweep rotating step."""
+    import gpe_hip
+    G, O = _pair(n, xmax)
+    phi = _phi(G, 11 + sum(n))
+    props = dict(gamma=gamma)
+    tg = _make(G, gpe_hip, phi, **props)
+    to = _make(O, oracle, phi, **props)
+    t4 = _four_sweep(lambda: _make(G, gpe_hip, phi, **props))
+    for t in (tg, to, t4):
+        t.omega = 0.4
+    pg = tg.solve_split(0.002, 10, 2)
+    po = to.solve_split(0.002, 10, 2)
+    p4 = _four_sweep(lambda: t4.solve_split(0.002, 10, 2))
+    e_o, e_4 = rel_l2(pg, po), rel_l2(pg, p4)
+    print(f"y-split rotating {n} gamma={gamma}: vs oracle {e_o:.3e}, vs four sweeps {e_4:.3e}", flush=True)
+    assert e_o < TOL and e_4 < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=TOL, atol=0)
+    dev = tg._device()
+    dev.profile_enable(True)
+    tg.solve_split(0.002, 10, 3)
+    prof = dev.profile_read()
+    dev.profile_enable(False)
+    seqs = {s for (s, _ax) in prof}
+    assert {20, 27, 28, 29} <= seqs and not ({12, 13, 14, 0, 1, 2} & seqs), seqs
