@@ -249,10 +249,13 @@ def main():
                 "alg_bytes_per_launch": dom["alg_bytes"],
                 "traffic_source": f"{tsrc} (FETCH_SIZE/WRITE_SIZE, copy-calibrated, committed)" if traffic else None}
     if dom["kernel"] == "ymid_x":
-        # what the counters show for this sweep (profiles/sweep256_c2_counters.md)
-        roofline["limiter"] = ("L2 request rate: each lane moves 16-byte pieces of lines shared by 8 tiles "
-                               "(8x the read requests of a line-coalesced sweep, TA busy 71 %); the two-sweep "
-                               "step it belongs to is faster than the four-sweep step at 0.77 of peak")
+        # what the layout measurements and counters show for this sweep
+        # (profiles/ysplit256_c2_two_sweep.md, profiles/sweep256_c2_counters.md)
+        roofline["limiter"] = ("latency at 2 waves per SIMD (245 VGPRs) plus partial lines: a tile holds one "
+                               "y residue a, so in block order each line it loads or stores carries 2 of its "
+                               "values (the line-coalesced r-major order runs this sweep at 144 us but its z "
+                               "sweep at 257 us); the two-sweep step it belongs to moves 72 B/pt against the "
+                               "four sweeps' 136 B/pt")
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
 
     # ------------------------------------------------- host-inclusive drop-in call

@@ -91,6 +91,8 @@ int gpe_create(const gpe_grid_desc* d, gpe_handle** out) {
     const long long nb = (h->ax[0].ncols + C2 - 1) / C2;
     if (nb > maxwg) maxwg = (int)nb;
   }
+  // the y-split x sweeps (3D, ny = 256) run 16 nz work-groups, one partial each
+  if (h->ndims == 3 && h->ny == 256 && 16 * h->nz > maxwg) maxwg = 16 * h->nz;
   h->maxwg = maxwg;
 
   auto bail = [&](int code, const char* what) {
@@ -153,6 +155,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->rtab_x);
   (void)hipFree(h->rtab_y);
   (void)hipFree(h->Vp);
+  (void)hipFree(h->psiy);
   (void)hipFree(h->kcolB);
   (void)hipFree(h->twy);
   (void)hipFree(h->cpl);

@@ -127,6 +127,7 @@ struct gpe_handle {
   // per z pencil in the split's spectral order; v_gen counts writes of V[0]
   bool ysplit_off = false;      // GPE_YSPLIT=0 turns the two-sweep step off
   double* Vp = nullptr;
+  double2* psiy = nullptr;       // y-split step: the state in block order inside a block
   double2* kcolB = nullptr;
   double2* twy = nullptr;       // 256-point twiddles of the y axis
   long long v_gen = 0, vp_gen = -1;
@@ -423,6 +424,7 @@ inline int ysplit_prepare(gpe_handle* h) {
   }
   if (!h->kcolB) HIPCHK(h, hipMalloc(&h->kcolB, (size_t)h->nx * h->ny * sizeof(double2)));
   if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
+  if (!h->psiy) HIPCHK(h, hipMalloc(&h->psiy, (size_t)h->npts * sizeof(double2)));
   return ysplit_refresh_v(h);
 }
 
@@ -476,11 +478,12 @@ inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
 
 // The y-split 3D step applies: one component, no rotation, ny = 256, x length with a
 // y-split x sweep, z length with a 16-pencil-multiple tile, nz a multiple of 8.
-inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy; }
+inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy && h->psiy; }
 
 // x-sweep parameters of the y-split step.
 inline SweepParams ysplit_params(gpe_handle* h) {
   SweepParams p = base_params(h, 0, h->psi[0], h->Vp);
+  p.psiy = h->psiy;
   p.twy = h->twy;
   p.gnx = (unsigned)h->nx;
   p.gny = (unsigned)h->ny;
@@ -492,12 +495,15 @@ inline SweepParams ysplit_params(gpe_handle* h) {
 inline SweepParams ysplit_zparams(gpe_handle* h) {
   SweepParams p = base_params(h, 2, h->psi[0], h->V[0]);
   p.kcol = h->kcolB;
+  p.psiy = h->psiy;
   p.twy = h->twy;
   return p;
 }
 
 // One y-split x sweep, timed like the other sweeps when profiling.
 inline int ysplit_xsweep(gpe_handle* h, int seq, const SweepParams& px, int* nb) {
+  if (16 * h->nz > h->maxwg)
+    return fail(h, GPE_EINVAL, "y-split x sweep: %d work-groups exceed the partial-sum pitch %d", 16 * h->nz, h->maxwg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof) {
     e0 = prof_event(h);

@@ -18,6 +18,42 @@ __device__ __forceinline__ int bitrev4(int r) {
   return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3);
 }
 
+// Storage order of the 256 y values of every (x, z) column while a block of the
+// two-sweep step runs (written by its opening x sweep, read by its closing one; at
+// block ends the state is in natural order y = a + 16 r).  An x-sweep tile holds one a
+// and every r, a z-sweep tile every a and one r, so no order puts a whole 128-byte line
+// (8 values) into one tile of both sweeps.  Bit i of YMASK says whether bit i of the
+// position within a line is the next bit of r (1) or of a (0); the lines then follow
+// in the order of the remaining bits of a, then of r (YMASK = 0 is the natural order).
+// YMASK = 2 (bits a0 r0 a1): a line holds a 4-value block of a times two r, so an x
+// tile touches each of its lines at two values of one 64-byte half and a z tile at two
+// pairs of neighbours.  Measured at 256^3 over the eight masks' classes: the natural
+// order gives 210 + 105 us per step (x sweep + z sweep), 4 a per line with a fastest
+// 189 + 118, r fastest 161 + 126, this order 161.5 + 121; 2 a per line 147 + 185 or
+// 152 + 160, 8 r per line 144 + 257.
+constexpr unsigned YMASK = 2u;
+constexpr unsigned YR_BITS = (YMASK & 1u) + ((YMASK >> 1) & 1u) + ((YMASK >> 2) & 1u), YA_BITS = 3u - YR_BITS;
+static_assert(YMASK < 8u, "a line holds 8 values");
+
+__device__ __forceinline__ unsigned ysplit_pos(unsigned a, unsigned r) {
+  unsigned off = 0u, ia = 0u, ir = 0u;
+#pragma unroll
+  for (unsigned i = 0; i < 3u; ++i) {
+    if ((YMASK >> i) & 1u) off |= ((r >> ir++) & 1u) << i;
+    else off |= ((a >> ia++) & 1u) << i;
+  }
+  return off + 8u * ((a >> YA_BITS) + (16u >> YA_BITS) * (r >> YR_BITS));
+}
+
+// Tile of work-group b (of nb) in the y-split z sweep, whose tiles take per = 256 / C
+// consecutive tiles for each x.  The 8 XCDs get the work-groups round robin; the per
+// tiles of one x, which share the lines of the block order, go to one XCD (nx is a
+// multiple of 8).
+__device__ __forceinline__ unsigned ysplit_ztile(unsigned b, unsigned per) {
+  const unsigned s = b >> 3;
+  return ((s / per) * 8u + (b & 7u)) * per + s % per;
+}
+
 // A tile of C pencils of length N (256 threads, T = N/16 threads per pencil, 16
 // elements each) is G = C/16 blocks of 16 pencils.  It moves between the canonical
 // pencil layout -- thread (c = tid % C, t = tid / C) holds element t + T m of pencil

@@ -122,6 +122,9 @@ struct SweepParams {
   // S_FWD / S_INV / S_KIN of sweep_kernel: a second array of the same shape, swept by
   // the second half of the grid with the same tables (null: one array)
   double2* psi2;
+  // y-split 3D steps: the state in block order, from a block's opening x sweep (which
+  // reads psi) to its closing one (which writes psi); the z sweeps run on it in place
+  double2* psiy;
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

@@ -93,7 +93,12 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const unsigned cgu = tile * (unsigned)C + (unsigned)c;
   const bool active = cgu < p.ncols;
   const unsigned cg = active ? cgu : p.ncols - 1u;
-  const unsigned base = (cg % p.A) * p.sa + (cg / p.A) * p.sb + (unsigned)t * p.stride;
+  unsigned col = (cg % p.A) * p.sa + (cg / p.A) * p.sb;
+  if constexpr (SEQ == S_YKIN || SEQ == S_YKINM) {
+    // z pencils of the y-split step: column y + 256 x, y = a + 16 r stored in block order
+    col = (col & ~255u) + ysplit_pos(col & 15u, (col >> 4) & 15u);
+  }
+  const unsigned base = col + (unsigned)t * p.stride;
   const unsigned step = (unsigned)T * p.stride;
 
   double2 v[E];
@@ -355,7 +360,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
   __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM ? 2 * N : N];
   __shared__ double red[4];
   unsigned bid = blockIdx.x, nbk = gridDim.x;
-  double2* psi = p.psi;
+  double2* psi = SEQ == S_YKIN || SEQ == S_YKINM ? p.psiy : p.psi;
   if (p.psi2) {   // two arrays: the second half of the grid sweeps psi2 (no reductions)
     nbk >>= 1;
     if (bid >= nbk) {
@@ -363,7 +368,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
       psi = p.psi2;
     }
   }
-  const unsigned tile = C < 16 ? xcd_pair_tile(bid, nbk) : bid;
+  const unsigned tile = SEQ == S_YKIN || SEQ == S_YKINM ? ysplit_ztile(bid, 256u / C)
+                        : C < 16                       ? xcd_pair_tile(bid, nbk)
+                                                       : bid;
   sweep_body<N, C, COLFAST, SEQ, VAR>(p, psi, tile, tile, lds, twl, red);
 }
 

@@ -10,7 +10,10 @@
 // q = x holds the 16 b slots of one (a, x), row_exchange16), where the y factor over b
 // is a 16-point DFT in registers and the phase step applies.  Between the sweeps the b
 // slot r holds the spectral index k1 = bitrev4(r), times the twiddle W256^{a k1} of the
-// 16 x 16 split; the z sweep applies the factor over a.
+// 16 x 16 split; the z sweep applies the factor over a.  From the opening x sweep of
+// a block to its closing one, the state lives in a second array (psiy), where the 256
+// values of every (x, z) column are in the block order ysplit_pos(a, r) (rowdft_dev.h)
+// instead of y = a + 16 r.
 //
 // The potential is read in row order (launch_ysplit_permute_v): for each b, the nx
 // values of the tile's rows are contiguous.
@@ -68,15 +71,24 @@ ysplit_x_kernel(const SweepParams p) {
   const unsigned ar = s8 % 16u, z = (s8 / 16u) * 8u + xcd;
   const unsigned ny = p.gny, nx = p.gnx;
   // pencil layout
+  constexpr bool NAT_IN = SEQ == S_YOPEN || SEQ == S_YSOPEN || SEQ == S_YROPEN;
+  constexpr bool NAT_OUT = SEQ == S_YCLOSEM || SEQ == S_YRCLOSEM;
   const int c = tid % C, t = tid / C;
-  const unsigned base = ar + 16u * (unsigned)c + ny * (unsigned)t + ny * nx * z;
+  const unsigned colz = ny * (unsigned)t + ny * nx * z;
+  // y-split 3D steps: the opening sweeps read psi in natural order and the closing ones
+  // write it; everything in between is in block order in psiy.  (In place, the two
+  // orders would race: a tile's natural positions are block positions of other tiles.)
+  const double2* src = NAT_IN ? p.psi : p.psiy;
+  double2* dst = NAT_OUT ? p.psi : p.psiy;
+  const unsigned base_in = (NAT_IN ? ar + 16u * (unsigned)c : ysplit_pos(ar, (unsigned)c)) + colz;
+  const unsigned base_out = (NAT_OUT ? ar + 16u * (unsigned)c : ysplit_pos(ar, (unsigned)c)) + colz;
   const unsigned step = (unsigned)T * ny;
   // row layout: row q = tid is x = tid of the tile's (a, z)
   const unsigned vrow = (z * 16u + ar) * 16u * (unsigned)NT + (unsigned)tid;
 
   double2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = p.psi[base + (unsigned)m * step];
+  for (int m = 0; m < E; ++m) v[m] = src[base_in + (unsigned)m * step];
   // the potential is loaded with the state, before the transform: at two waves per SIMD
   // nothing else would hide its latency at the phase
   // (rotating frame: in the pencil layout, where its phase applies; the row order keeps
@@ -142,12 +154,12 @@ ysplit_x_kernel(const SweepParams p) {
       for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * (unsigned)k1]);
       row_exchange16<N, C, false>(v, lds);
 #pragma unroll
-      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
     } else {
       // block-end sums of the stored real-space state (solve_split.m:65-77), the x
       // kinetic term with the rotating-frame cross term of the pencil's y
 #pragma unroll
-      for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
       double accA = 0.0, accB = 0.0, accC = 0.0, accK = 0.0;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -223,7 +235,7 @@ ysplit_x_kernel(const SweepParams p) {
     row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
 #pragma unroll
-    for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+    for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
   } else {
     // block-end reductions of the real-space state (solve_split.m:65-77), which is
     // stored with slot r = b
@@ -238,7 +250,7 @@ ysplit_x_kernel(const SweepParams p) {
     }
     row_exchange16<N, C, false>(v, lds);
 #pragma unroll
-    for (int m = 0; m < E; ++m) p.psi[base + (unsigned)m * step] = v[m];
+    for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
     fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);

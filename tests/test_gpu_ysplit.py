@@ -88,7 +88,7 @@ def _four_sweep(fn):
 
 
 CASES = [((64, 256, 64), (6, 9, 5)), ((128, 256, 64), (7, 10, 5)), ((64, 256, 128), (6, 8, 7)),
-         ((256, 256, 64), (9, 9, 5))]
+         ((256, 256, 64), (9, 9, 5)), ((64, 256, 256), (6, 9, 9))]
 
 
 @pytest.mark.parametrize("n,xmax", CASES)
