@@ -30,6 +30,23 @@ __device__ __forceinline__ double2 mul_i(double2 a) { return make_double2(-a.y, 
 __device__ __forceinline__ double2 mul_mi(double2 a) { return make_double2(a.y, -a.x); }  // -i*a
 __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 
+// An opaque copy: addresses derived from it are recomputed where they are used
+// instead of being kept live from the kernel's first loads.
+__device__ __forceinline__ unsigned opaque(unsigned x) {
+  unsigned y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
+// The value of the partner lane (lane ^ 1): DPP quad_perm [1, 0, 3, 2].
+__device__ __forceinline__ double xchg(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double2 xchg(double2 v) { return make_double2(xchg(v.x), xchg(v.y)); }
+
 // cos(2*pi*j/48) for j = 0..12 (first quadrant), correctly rounded decimal literals.
 __device__ __forceinline__ double q48(int j) {
   switch (j) {

@@ -68,22 +68,6 @@ __device__ __forceinline__ void rabi_taylor(double h, double z, double* cs, doub
   *f = h * fma(s, z, 1.0);
 }
 
-// An opaque copy: addresses derived from it are recomputed where they are used
-// instead of being kept live from the kernel's first loads.
-__device__ __forceinline__ unsigned opaque(unsigned x) {
-  unsigned y;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
-  return y;
-}
-
-// The value of the partner lane (lane ^ 1): DPP quad_perm [1, 0, 3, 2].
-__device__ __forceinline__ double xchg(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0xB1, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ double2 xchg(double2 v) { return make_double2(xchg(v.x), xchg(v.y)); }
 
 // Coupling kinds of SP_PR, fixed per launch.
 // CK_ARRAY_REAL: array Omega with dt = i di; CK_ARRAY_TAYLOR: the same with
