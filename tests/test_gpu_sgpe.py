@@ -94,3 +94,19 @@ def test_sgpe_two_sweep_step_on_a_256_point_y_axis(ecut):
     print(f"SGPE two-sweep 64x256x64 ecut={ecut}: vs oracle {e_o:.2e}, vs four-sweep {e_4:.2e}", flush=True)
     assert e_o < 1e-11 and e_4 < 1e-12
     assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
+
+
+def test_sgpe_two_sweep_step_with_more_x_tiles_than_resident():
+    """64 x 256 x 256: the x sweeps run 4096 work-groups, more than the GPU holds at once,
+    so the tiles of one z plane do not all run together; the block's opening and
+    closing sweeps must still move the state between its two orders without a tile
+    reading another's stores.  Same noise stream as the four-sweep step."""
+    n, xmax = (64, 256, 256), (6, 8, 8)
+    tg, _ = _with_schedule("1", lambda: _pair(3, n, xmax, ecut=6.0))
+    pg = _with_schedule("1", lambda: tg.solve_split(0.005, 10, 2))
+    t4, _ = _with_schedule("0", lambda: _pair(3, n, xmax, ecut=6.0))
+    p4 = _with_schedule("0", lambda: t4.solve_split(0.005, 10, 2))
+    e_4 = rel_l2(pg, p4)
+    print(f"SGPE two-sweep 64x256x256: vs four-sweep {e_4:.2e}", flush=True)
+    assert e_4 < 1e-12
+    assert np.allclose(tg.history["n"], t4.history["n"], rtol=1e-11, atol=0)

@@ -210,3 +210,23 @@ def test_ysplit_rotating_frame_matches_oracle_and_four_sweeps(n, xmax, gamma):
     dev.profile_enable(False)
     seqs = {s for (s, _ax) in prof}
     assert {20, 27, 28, 29} <= seqs and not ({12, 13, 14, 0, 1, 2} & seqs), seqs
+
+
+def test_ysplit_rotating_frame_with_more_x_tiles_than_resident():
+    """64 x 256 x 256 (4096 x-sweep work-groups): the rotating-frame opening and closing
+    x sweeps convert between the natural and the block order through the second array;
+    the run equals the library's four-sweep rotating step."""
+    import gpe_hip
+    n, xmax = (64, 256, 256), (6, 9, 9)
+    G, _ = _pair(n, xmax)
+    phi = _phi(G, 17)
+    tg = _make(G, gpe_hip, phi, gamma=0.02)
+    t4 = _four_sweep(lambda: _make(G, gpe_hip, phi, gamma=0.02))
+    for t in (tg, t4):
+        t.omega = 0.4
+    pg = tg.solve_split(0.002, 10, 2)
+    p4 = _four_sweep(lambda: t4.solve_split(0.002, 10, 2))
+    e_4 = rel_l2(pg, p4)
+    print(f"y-split rotating 64x256x256: vs four sweeps {e_4:.3e}", flush=True)
+    assert e_4 < TOL
+    assert np.allclose(tg.history["mu"], t4.history["mu"], rtol=TOL, atol=0)
