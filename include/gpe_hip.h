@@ -117,7 +117,9 @@ int gpe_set_coupling(gpe_handle* h, const double* omega, int is_array);
 /* solve_split prologue (lib/@GPEtask/solve_split.m:13-46;
  * lib/@GPEtaskSpinor/solve_split.m:13-57): NN0, n_rec fallback, complex dt,
  * kinetic operator, initial chemical potential.  muc_out receives muc (the
- * value the reference computes at line 41). */
+ * value the reference computes at line 41).  On a one-component 3D grid with a
+ * 256-point y axis the step runs as two sweeps; the first such call allocates their
+ * work arrays (npts complex values and npts doubles, kept until gpe_destroy). */
 int gpe_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_out);
 
 /* One iteration j of the outer loop with a static potential
