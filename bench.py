@@ -251,11 +251,11 @@ def main():
     if dom["kernel"] == "ymid_x":
         # what the layout measurements and counters show for this sweep
         # (profiles/ysplit256_c2_two_sweep.md, profiles/sweep256_c2_counters.md)
-        roofline["limiter"] = ("latency at 2 waves per SIMD (245 VGPRs) plus partial lines: a tile holds one "
-                               "y residue a, so in block order each line it loads or stores carries 2 of its "
-                               "values (the line-coalesced r-major order runs this sweep at 144 us but its z "
-                               "sweep at 257 us); the two-sweep step it belongs to moves 72 B/pt against the "
-                               "four sweeps' 136 B/pt")
+        roofline["limiter"] = ("its own instruction stream at 2 waves per SIMD (245 VGPRs): with its tiles "
+                               "L2-resident it still takes 138 us; HBM and partial lines add the rest (a tile "
+                               "holds one y residue a, so each line it loads or stores carries 2 of its values; "
+                               "a line-coalesced order runs it at 144 us but the z sweep at 257 us). The "
+                               "two-sweep step it belongs to moves 72 B/pt against the four sweeps' 136 B/pt")
     step_gbps = STEP_BYTES_3D * npts * steps_per_s / 1e9
 
     # ------------------------------------------------- host-inclusive drop-in call

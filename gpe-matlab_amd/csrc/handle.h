@@ -476,8 +476,8 @@ inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
   return GPE_OK;
 }
 
-// The y-split 3D step applies: one component, no rotation, ny = 256, x length with a
-// y-split x sweep, z length with a 16-pencil-multiple tile, nz a multiple of 8.
+// The y-split 3D step applies: one component, ny = 256, nx and nz in {64, 128, 256},
+// nz a multiple of 8 (ysplit_eligible), with its tables and work arrays in place.
 inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy && h->psiy; }
 
 // x-sweep parameters of the y-split step.

@@ -18,9 +18,10 @@ __device__ __forceinline__ int bitrev4(int r) {
   return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3);
 }
 
-// Storage order of the 256 y values of every (x, z) column while a block of the
-// two-sweep step runs (written by its opening x sweep, read by its closing one; at
-// block ends the state is in natural order y = a + 16 r).  An x-sweep tile holds one a
+// Storage order of the 256 y values of every (x, z) column in the second state array
+// of the two-sweep step, which holds the state while a block runs (written by the
+// block's opening x sweep, read by its closing one; at block ends psi holds the state
+// in natural order y = a + 16 r).  An x-sweep tile holds one a
 // and every r, a z-sweep tile every a and one r, so no order puts a whole 128-byte line
 // (8 values) into one tile of both sweeps.  Bit i of YMASK says whether bit i of the
 // position within a line is the next bit of r (1) or of a (0); the lines then follow
@@ -45,10 +46,10 @@ __device__ __forceinline__ unsigned ysplit_pos(unsigned a, unsigned r) {
   return off + 8u * ((a >> YA_BITS) + (16u >> YA_BITS) * (r >> YR_BITS));
 }
 
-// Tile of work-group b (of nb) in the y-split z sweep, whose tiles take per = 256 / C
-// consecutive tiles for each x.  The 8 XCDs get the work-groups round robin; the per
-// tiles of one x, which share the lines of the block order, go to one XCD (nx is a
-// multiple of 8).
+// Tile of work-group b in the y-split z sweep, which has per = 256 / C tiles for each x
+// (C pencils per tile) and nx per work-groups.  The 8 XCDs get the work-groups round
+// robin; the per tiles of one x, which share the lines of the block order, go to one
+// XCD (nx is a multiple of 8).
 __device__ __forceinline__ unsigned ysplit_ztile(unsigned b, unsigned per) {
   const unsigned s = b >> 3;
   return ((s / per) * 8u + (b & 7u)) * per + s % per;
