@@ -331,6 +331,43 @@ __device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, doub
   }
 }
 
+// The same exchange with real and imaginary parts moved together (16-byte LDS accesses,
+// element i of column c at i * C + c): half the LDS instructions and barriers, twice the
+// footprint (N C complex values).
+template <class PL, int p, int C>
+__device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, double2* lds) {
+  constexpr int R = PL::radix(p);
+  constexpr int B = PL::E / R;
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b)
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[pass_out_index<PL, p>(t, b, r) * C + c] = v[b + r * B];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < PL::E; ++m) v[m] = lds[(t + PL::T * m) * C + c];
+}
+
+// Full transform of one pencil held in canonical layout, complex LDS exchanges (columns
+// consecutive across lanes).
+template <class PL, int DIR, int C>
+__device__ __forceinline__ void fft_pencil(double2 (&v)[PL::E], int t, int c, double2* lds,
+                                           const double2* __restrict__ tw) {
+  stockham_pass<PL, 0, DIR>(v, t, tw);
+  if constexpr (PL::P > 1) {
+    exchange<PL, 0, C>(v, t, c, lds);
+    stockham_pass<PL, 1, DIR>(v, t, tw);
+  }
+  if constexpr (PL::P > 2) {
+    exchange<PL, 1, C>(v, t, c, lds);
+    stockham_pass<PL, 2, DIR>(v, t, tw);
+  }
+  if constexpr (PL::P > 3) {
+    exchange<PL, 2, C>(v, t, c, lds);
+    stockham_pass<PL, 3, DIR>(v, t, tw);
+  }
+}
+
 // Full transform of one pencil held in canonical layout.
 template <class PL, int DIR, int C, bool COLFAST>
 __device__ __forceinline__ void fft_pencil(double2 (&v)[PL::E], int t, int c, double* lds,

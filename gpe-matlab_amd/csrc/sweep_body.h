@@ -64,6 +64,17 @@ __device__ __forceinline__ void table_kin(double2 (&v)[PL::E], int t, int c, dou
   fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
 }
 
+// The same with complex LDS exchanges (columns consecutive across lanes).
+template <class PL, int C>
+__device__ __forceinline__ void table_kin(double2 (&v)[PL::E], int t, int c, double2* lds, const double2* twl,
+                                          const double2* __restrict__ row) {
+  constexpr int T = PL::T, E = PL::E;
+  fft_pencil<PL, -1, C>(v, t, c, lds, twl);
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmul(row[t + T * m], v[m]);
+  fft_pencil<PL, +1, C>(v, t, c, lds, twl);
+}
+
 // Kinetic-energy weight of index i: k_i^2/2/n, plus the rotating-frame cross term.
 __device__ __forceinline__ double ke_weight(const SweepParams& p, int i, double rc) {
   return p.rot_w != 0.0 ? fma(p.rot_w * rc, p.k1ax[i], p.keax[i]) : p.keax[i];

@@ -57,7 +57,11 @@ ysplit_x_kernel(const SweepParams p) {
   constexpr bool OPEN = SEQ == S_YOPEN || SEQ == S_YSOPEN, NOISE = SEQ == S_YSOPEN || SEQ == S_YSMID;
   constexpr bool ROT = SEQ == S_YROPEN || SEQ == S_YRMID || SEQ == S_YRCLOSEM;
   static_assert(E == 16 && NT % 64 == 0, "16 elements per thread, whole wavefronts");
-  __shared__ double lds[Lds<N, C, true>::SIZE];
+  // LDS exchanges move real and imaginary parts together (16-byte accesses): half the
+  // LDS instructions and barriers of the split exchanges of the other sweeps.  At two
+  // groups per CU (the register limit) the 72 KB per group fit; measured at 256^3 the
+  // mid-block sweep took 155 us against 161 us with split exchanges.
+  __shared__ double2 lds[Lds<N, C, true>::SIZE];
   __shared__ double2 twl[N];
   __shared__ double2 twy[256];
   __shared__ double red[NW];
@@ -122,7 +126,7 @@ ysplit_x_kernel(const SweepParams p) {
         const double aa = -(vv[m] - mu) - p.g * cabs2(x);
         v[m] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);   // solve_split.m:55-56
       }
-      table_kin<PL, C, true>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:23
+      table_kin<PL, C>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:23
     } else {
       row_exchange16<N, C, true>(v, lds);
 #pragma unroll
@@ -131,7 +135,7 @@ ysplit_x_kernel(const SweepParams p) {
 #pragma unroll
       for (int b = 0; b < 16; ++b) v[b] = u[b];
       row_exchange16<N, C, false>(v, lds);
-      table_kin<PL, C, true>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:25
+      table_kin<PL, C>(v, t, c, lds, twl, rrow);                   // ssft_kin_step.m:25
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const double vm = vv[m] - mu;
@@ -143,7 +147,7 @@ ysplit_x_kernel(const SweepParams p) {
         }
         v[m] = x;
       }
-      if constexpr (SEQ == S_YRMID) table_kin<PL, C, true>(v, t, c, lds, twl, rrow);   // next step's 23
+      if constexpr (SEQ == S_YRMID) table_kin<PL, C>(v, t, c, lds, twl, rrow);   // next step's 23
     }
     if constexpr (SEQ != S_YRCLOSEM) {
       row_exchange16<N, C, true>(v, lds);
@@ -168,7 +172,7 @@ ysplit_x_kernel(const SweepParams p) {
         accB += vv[m] * n;
         accC += n * n;
       }
-      fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
+      fft_pencil<PL, -1, C>(v, t, c, lds, twl);
       const double rc = p.rcoord[yrow];
 #pragma unroll
       for (int m = 0; m < E; ++m) accK += ke_weight(p, t + T * m, rc) * cabs2(v[m]);
@@ -199,7 +203,7 @@ ysplit_x_kernel(const SweepParams p) {
       u[b] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);
     }
   } else {
-    fft_pencil<PL, +1, C, true>(v, t, c, lds, twl);
+    fft_pencil<PL, +1, C>(v, t, c, lds, twl);
     row_exchange16<N, C, true>(v, lds);
     // undo the split twiddle and the y factor over b left by the previous x sweep
 #pragma unroll
@@ -233,7 +237,7 @@ ysplit_x_kernel(const SweepParams p) {
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) v[br4c(k1)] = cmul(u[k1], twy[ar * (unsigned)k1]);
     row_exchange16<N, C, false>(v, lds);
-    fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
+    fft_pencil<PL, -1, C>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
   } else {
@@ -251,7 +255,7 @@ ysplit_x_kernel(const SweepParams p) {
     row_exchange16<N, C, false>(v, lds);
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[base_out + (unsigned)m * step] = v[m];
-    fft_pencil<PL, -1, C, true>(v, t, c, lds, twl);
+    fft_pencil<PL, -1, C>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) accK += p.keax[t + T * m] * cabs2(v[m]);
     const double sa = block_sum_w<NW>(accA, red);
