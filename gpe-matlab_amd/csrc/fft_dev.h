@@ -331,39 +331,39 @@ __device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, doub
   }
 }
 
-// The same exchange with real and imaginary parts moved together (16-byte LDS accesses,
-// element i of column c at i * C + c): half the LDS instructions and barriers, twice the
+// The same exchange with real and imaginary parts moved together (16-byte LDS accesses at
+// the same slots, XOR-ed with sw): half the LDS instructions and barriers, twice the
 // footprint (N C complex values).
-template <class PL, int p, int C>
-__device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, double2* lds) {
+template <class PL, int p, int C, bool COLFAST>
+__device__ __forceinline__ void exchange(double2 (&v)[PL::E], int t, int c, double2* lds, unsigned sw = 0) {
+  using L = Lds<PL::N, C, COLFAST>;
   constexpr int R = PL::radix(p);
   constexpr int B = PL::E / R;
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < B; ++b)
 #pragma unroll
-    for (int r = 0; r < R; ++r) lds[pass_out_index<PL, p>(t, b, r) * C + c] = v[b + r * B];
+    for (int r = 0; r < R; ++r) lds[L::at(c, pass_out_index<PL, p>(t, b, r)) ^ sw] = v[b + r * B];
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < PL::E; ++m) v[m] = lds[(t + PL::T * m) * C + c];
+  for (int m = 0; m < PL::E; ++m) v[m] = lds[L::at(c, t + PL::T * m) ^ sw];
 }
 
-// Full transform of one pencil held in canonical layout, complex LDS exchanges (columns
-// consecutive across lanes).
-template <class PL, int DIR, int C>
+// Full transform of one pencil held in canonical layout, complex LDS exchanges.
+template <class PL, int DIR, int C, bool COLFAST = true>
 __device__ __forceinline__ void fft_pencil(double2 (&v)[PL::E], int t, int c, double2* lds,
-                                           const double2* __restrict__ tw) {
+                                           const double2* __restrict__ tw, unsigned sw = 0) {
   stockham_pass<PL, 0, DIR>(v, t, tw);
   if constexpr (PL::P > 1) {
-    exchange<PL, 0, C>(v, t, c, lds);
+    exchange<PL, 0, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 1, DIR>(v, t, tw);
   }
   if constexpr (PL::P > 2) {
-    exchange<PL, 1, C>(v, t, c, lds);
+    exchange<PL, 1, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 2, DIR>(v, t, tw);
   }
   if constexpr (PL::P > 3) {
-    exchange<PL, 2, C>(v, t, c, lds);
+    exchange<PL, 2, C, COLFAST>(v, t, c, lds, sw);
     stockham_pass<PL, 3, DIR>(v, t, tw);
   }
 }

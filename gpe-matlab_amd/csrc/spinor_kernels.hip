@@ -23,6 +23,8 @@
 //         x-FFT of both adjoint components, the transposed step linearised at the
 //         stored pre-step state, forward x-FFT, and the per-work-group sums of
 //         dL/d om0 and dL/d kw.
+#include <type_traits>
+
 #include "fft_dev.h"
 #include "phase_dev.h"
 #include "reduce_dev.h"
@@ -88,7 +90,14 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   using L = Lds<N, C2, true>;
   constexpr int T = PL::T, E = PL::E;
   constexpr bool COLFAST = true;
-  __shared__ double lds[2 * L::SIZE];
+  // The transposed step of the adjoint moves real and imaginary parts together through
+  // LDS (half the LDS instructions and barriers; 2 x 24 KB at 192): measured at 192^3 it
+  // spills 48 VGPRs instead of 70 and runs 9 % faster (352 -> 320 us per launch).  The forward step (same speed
+  // either way, c5: 2437-2445 against 2448-2453 steps/s) and the block-end sums keep the
+  // split exchanges, and so do the imaginary-time sweeps, which would lose a wave per
+  // SIMD (registers and LDS).
+  constexpr bool CPLX_LDS = SEQ == SP_ADJ;
+  __shared__ typename std::conditional<CPLX_LDS, double2, double>::type lds[2 * L::SIZE];
   __shared__ double2 twl[N];
   __shared__ double2 exl[SEQ == SP_ADJ ? N : 1];   // SP_ADJ: exp(i kw x) per x index
   __shared__ double xl[SEQ == SP_ADJ ? N : 1];     // SP_ADJ: x per x index
@@ -99,7 +108,7 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   const int q = tid >> 1;
   const int c = q % C2;
   const int t = q / C2;
-  double* ldsk = lds + k * L::SIZE;
+  auto* ldsk = lds + k * L::SIZE;
   // Lanes 2i and 2i+1 address the same slot of their component's half of LDS, and the
   // halves are whole 64-bank lines apart, so every exchange access would be a 2-way bank
   // conflict.  Component 2 XORs its slot index with 28 instead: within a 32-slot line
@@ -108,7 +117,8 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   // of 16 and of 32 lanes alike.  Measured at 192^3: 608 -> 0 conflict cycles per wave,
   // the same VALU count.
   static_assert(L::SIZE % 32 == 0, "the slot XOR stays inside whole 32-slot lines");
-  const unsigned swz = k ? 28u : 0u;
+  // The adjoint's 16-byte exchanges run without it (320 us per launch at 192^3, 326 with).
+  const unsigned swz = (k && SEQ != SP_ADJ) ? 28u : 0u;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
   if constexpr (SEQ == SP_ADJ) {
     for (int i = tid; i < N; i += 256) {
