@@ -80,6 +80,61 @@ __global__ void __launch_bounds__(256) ham_acc_kernel(double2* __restrict__ out,
   }
 }
 
+__global__ void __launch_bounds__(256) ham_scale_kernel(double2* __restrict__ x, long long n, double s) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double2 v = x[i];
+    x[i] = make_double2(v.x * s, v.y * s);
+  }
+}
+
+// S_KIN tables of the kinetic term: kcol = 1 for every pencil, then per axis k^2/2/n_a
+// (off2) and, for the rotating frame, k/n_a of x and y (off1).
+std::vector<double2> kinetic_tables(const gpe_handle* h, bool rot, long long off2[3], long long off1[3]) {
+  long long maxcols = 0;
+  for (int a = 0; a < h->ndims; ++a) maxcols = std::max(maxcols, h->ax[a].ncols);
+  std::vector<double2> tab((size_t)maxcols, make_double2(1.0, 0.0));
+  for (int a = 0; a < 3; ++a) off2[a] = off1[a] = 0;
+  for (int a = 0; a < h->ndims; ++a) {
+    const Axis& A = h->ax[a];
+    off2[a] = (long long)tab.size();
+    for (int i = 0; i < A.n; ++i) tab.push_back(make_double2(A.k[i] * A.k[i] / 2 / A.n, 0.0));
+    off1[a] = (long long)tab.size();
+    if (rot && a < 2)
+      for (int i = 0; i < A.n; ++i) tab.push_back(make_double2(A.k[i] / A.n, 0.0));
+  }
+  return tab;
+}
+
+// out += lap(phi, omega) (grid2d/lap.m:5-9, grid3d/lap.m:5-10) on the device: one S_KIN
+// sweep per axis of a copy of phi in tmp (and per first derivative when omega != 0).
+int add_kinetic(gpe_handle* h, const double2* phi, double2* out, double2* tmp, const double2* dtab,
+                const long long off2[3], const long long off1[3], double omega) {
+  int rc;
+  const long long n = h->npts;
+  const int nb = elem_blocks(n);
+  const bool rot = omega != 0.0;
+  for (int a = 0; a < h->ndims; ++a) {
+    const int passes = rot && a < 2 ? 2 : 1;
+    for (int q = 0; q < passes; ++q) {
+      HIPCHK(h, hipMemcpyAsync(tmp, phi, (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, h->stream));
+      SweepParams sp = base_params(h, a, tmp, nullptr);
+      sp.kcol = dtab;
+      sp.kax = dtab + (q == 0 ? off2[a] : off1[a]);
+      sp.trow = 0;
+      sp.rtab = nullptr;
+      if ((rc = sweep(h, a, gpe::S_KIN, sp))) return rc;
+      // grid3d/lap.m:8-10: -omega y on the x derivative, +omega x on the y derivative
+      const int mode = q == 0 ? 0 : (a == 0 ? 1 : 2);
+      const double cw = a == 0 ? -omega : omega;
+      const double* coord = a == 0 ? h->ax[1].xc : h->ax[0].xc;
+      hipLaunchKernelGGL(ham_acc_kernel, dim3(nb), dim3(256), 0, h->stream, out, (const double2*)tmp, n, mode,
+                         coord, cw, (unsigned)h->ny, (unsigned)h->nx);
+      HIPCHK(h, hipGetLastError());
+    }
+  }
+  return GPE_OK;
+}
+
 }  // namespace
 
 extern "C" int gpe_applyham(gpe_handle* h, const double* phi, double* out, int h0_only) {
@@ -93,19 +148,8 @@ extern "C" int gpe_applyham(gpe_handle* h, const double* phi, double* out, int h
   const long long n = h->npts;
   const int nd = h->ndims;
   const bool rot = h->omega != 0.0;
-  // S_KIN tables: kcol = 1 for every pencil, then per axis k^2/2/n_a (and k/n_a when rotating)
-  long long maxcols = 0;
-  for (int a = 0; a < nd; ++a) maxcols = std::max(maxcols, h->ax[a].ncols);
-  std::vector<double2> tab((size_t)maxcols, make_double2(1.0, 0.0));
-  long long off2[3] = {0, 0, 0}, off1[3] = {0, 0, 0};
-  for (int a = 0; a < nd; ++a) {
-    const Axis& A = h->ax[a];
-    off2[a] = (long long)tab.size();
-    for (int i = 0; i < A.n; ++i) tab.push_back(make_double2(A.k[i] * A.k[i] / 2 / A.n, 0.0));
-    off1[a] = (long long)tab.size();
-    if (rot && a < 2)
-      for (int i = 0; i < A.n; ++i) tab.push_back(make_double2(A.k[i] / A.n, 0.0));
-  }
+  long long off2[3], off1[3];
+  const std::vector<double2> tab = kinetic_tables(h, rot, off2, off1);
   // work buffers: phi, the output and one sweep copy
   const size_t cb = (size_t)n * sizeof(double2);
   const size_t nphi = (size_t)nc;
@@ -148,28 +192,63 @@ extern "C" int gpe_applyham(gpe_handle* h, const double* phi, double* out, int h
       hipLaunchKernelGGL(ham_point_kernel, dim3(nb), dim3(256), 0, h->stream, pp);
       HIPCHK(h, hipGetLastError());
       // kinetic terms, one sweep per axis (and per derivative when rotating)
-      for (int a = 0; a < nd; ++a) {
-        const int passes = rot && a < 2 ? 2 : 1;
-        for (int q = 0; q < passes; ++q) {
-          HIPCHK(h, hipMemcpyAsync(tmp, ph[j], cb, hipMemcpyDeviceToDevice, h->stream));
-          SweepParams sp = base_params(h, a, tmp, nullptr);
-          sp.kcol = dtab;
-          sp.kax = dtab + (q == 0 ? off2[a] : off1[a]);
-          sp.trow = 0;
-          sp.rtab = nullptr;
-          if ((rc = sweep(h, a, gpe::S_KIN, sp))) return rc;
-          // grid3d/lap.m:8-10: -omega y on the x derivative, +omega x on the y derivative
-          const int mode = q == 0 ? 0 : (a == 0 ? 1 : 2);
-          const double cw = a == 0 ? -h->omega : h->omega;
-          const double* coord = a == 0 ? h->ax[1].xc : h->ax[0].xc;
-          hipLaunchKernelGGL(ham_acc_kernel, dim3(nb), dim3(256), 0, h->stream, oj, (const double2*)tmp, n, mode,
-                             coord, cw, (unsigned)h->ny, (unsigned)h->nx);
-          HIPCHK(h, hipGetLastError());
-        }
-      }
+      if ((rc = add_kinetic(h, ph[j], oj, tmp, dtab, off2, off1, h->omega))) return rc;
     }
     for (int j = 0; j < nc; ++j)
       HIPCHK(h, hipMemcpyAsync(out + 2 * (size_t)n * j, ho + (size_t)n * j, cb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GPE_OK;
+  };
+  rc = run();
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  return rc;
+}
+
+extern "C" int gpe_grid_transform(gpe_handle* h, int op, double omega, const double* in, double* out) {
+  if (!h) return GPE_EINVAL;
+  if (!in || !out) return fail(h, GPE_EINVAL, "gpe_grid_transform: null array");
+  if (op != GPE_GRID_FFT && op != GPE_GRID_IFFT && op != GPE_GRID_LAP)
+    return fail(h, GPE_EINVAL, "gpe_grid_transform: unknown op %d", op);
+  if (!std::isfinite(omega)) return fail(h, GPE_EINVAL, "gpe_grid_transform: omega is not finite");
+  HIPCHK(h, hipSetDevice(h->device));
+  int rc;
+  const long long n = h->npts;
+  const bool lap = op == GPE_GRID_LAP;
+  long long off2[3], off1[3];
+  const std::vector<double2> tab =
+      lap ? kinetic_tables(h, omega != 0.0, off2, off1) : std::vector<double2>();
+  // work buffers: the array (transformed in place, or phi of lap), lap's sum and sweep copy
+  const size_t cb = (size_t)n * sizeof(double2);
+  char* buf = nullptr;
+  const size_t bytes = (lap ? 3 : 1) * cb + tab.size() * sizeof(double2);
+  HIPCHK(h, hipMalloc(&buf, bytes));
+  auto run = [&]() -> int {
+    double2* x = (double2*)buf;
+    HIPCHK(h, hipMemcpyAsync(x, in, cb, hipMemcpyHostToDevice, h->stream));
+    const double2* res = x;
+    if (lap) {
+      double2* acc = (double2*)(buf + cb);
+      double2* tmp = (double2*)(buf + 2 * cb);
+      double2* dtab = (double2*)(buf + 3 * cb);
+      HIPCHK(h, hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemsetAsync(acc, 0, cb, h->stream));
+      if ((rc = add_kinetic(h, x, acc, tmp, dtab, off2, off1, omega))) return rc;
+      res = acc;
+    } else {
+      // fftn / ifftn (grid2d/fft.m:11, ifft.m:11; grid3d/fft.m:11, ifft.m:11): one
+      // transform sweep per axis; ifftn divides by the number of points
+      for (int a = 0; a < h->ndims; ++a) {
+        SweepParams sp = base_params(h, a, x, nullptr);
+        sp.apply_scale = 0;
+        if ((rc = sweep(h, a, op == GPE_GRID_FFT ? gpe::S_FWD : gpe::S_INV, sp))) return rc;
+      }
+      if (op == GPE_GRID_IFFT) {
+        hipLaunchKernelGGL(ham_scale_kernel, dim3(elem_blocks(n)), dim3(256), 0, h->stream, x, n, 1.0 / (double)n);
+        HIPCHK(h, hipGetLastError());
+      }
+    }
+    HIPCHK(h, hipMemcpyAsync(out, res, cb, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return GPE_OK;
   };

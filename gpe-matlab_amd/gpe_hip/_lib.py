@@ -98,6 +98,7 @@ SIGNATURES = {
     "gpe_split_block": [_P, _D, _DP, _DP],
     "gpe_measure": [_P, _DP],
     "gpe_applyham": [_P, _DP, _DP, C.c_int],
+    "gpe_grid_transform": [_P, C.c_int, C.c_double, _DP, _DP],
     "gpe_itp": [_P, C.POINTER(ItpArgs), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_itp2": [_P, C.POINTER(Itp2Args), _DP, _DP, _DP, _IP, _DP, _IP],
     "gpe_device_info": [C.c_char_p, C.c_int],
@@ -357,6 +358,20 @@ class Device:
         self.call("gpe_applyham", None if src is None else src.view(np.float64).ctypes.data_as(_DP),
                   out.view(np.float64).ctypes.data_as(_DP), int(bool(h0_only)))
         return [out[j * n:(j + 1) * n].reshape(self.shape, order="F") for j in range(ncomp)]
+
+    GRID_OPS = {"fft": 0, "ifft": 1, "lap": 2}
+
+    def grid_transform(self, op, arr, omega=0.0):
+        """fftn / ifftn / lap(phi, omega) of an array of the grid's shape on the device
+        (lib/@grid2d and lib/@grid3d fft.m, ifft.m, lap.m); returns a C-order array."""
+        a = np.asarray(arr, dtype=np.complex128)
+        if a.shape != tuple(self.shape):
+            raise ValueError(f"array of shape {a.shape} on a grid of shape {tuple(self.shape)}")
+        src = np.ascontiguousarray(a.ravel(order="F"))
+        out = np.empty(self.npts, dtype=np.complex128)
+        self.call("gpe_grid_transform", int(self.GRID_OPS[op]), float(omega),
+                  src.view(np.float64).ctypes.data_as(_DP), out.view(np.float64).ctypes.data_as(_DP))
+        return out.reshape(self.shape, order="F")
 
     def split_begin(self, ddt, niter_inner):
         muc = C.c_double()

@@ -4,7 +4,9 @@ The grid objects hold the host-side mesh and k tables exactly as the reference
 builds them (init.m); the device copies of the k tables are built by the HIP
 library from the same (xmax, n) description.  The quadrature helpers below are
 host conveniences of the grid API (grid1d/integrate.m etc.); the split-step hot
-path never calls them.
+path never calls them.  fft, ifft and lap run on the GPU through the library's
+transform sweeps (gpe_grid_transform), on a device handle the grid creates when it
+first needs one.
 """
 from types import SimpleNamespace
 
@@ -33,6 +35,26 @@ class _Grid:
     def normalize(self, psi):
         # grid1d/normalize.m:13-15
         return psi * (1.0 / self.norm(psi))
+
+    def _device(self):
+        dev = getattr(self, "_dev", None)
+        if dev is None:
+            from ._lib import Device
+            dev = self._dev = Device(self)
+        return dev
+
+    def fft(self, fun):
+        """fftn(fun) (grid2d/fft.m:11, grid3d/fft.m:11), on the GPU."""
+        return self._device().grid_transform("fft", fun)
+
+    def ifft(self, ifun):
+        """ifftn(ifun) (grid2d/ifft.m:11, grid3d/ifft.m:11), on the GPU."""
+        return self._device().grid_transform("ifft", ifun)
+
+    def lap(self, phi, omega=0.0):
+        """lap(phi, omega) (grid2d/lap.m:5-9, grid3d/lap.m:5-10): ifftn(kk .* fftn(phi)),
+        or for omega != 0 the per-axis rotating-frame form, on the GPU."""
+        return self._device().grid_transform("lap", phi, omega)
 
 
 class grid2d(_Grid):

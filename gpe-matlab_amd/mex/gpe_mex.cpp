@@ -310,6 +310,17 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     mxArray* out = grid_complex(t, nc);
     check(gpe_applyham(t.h, in, cdata(out), as_int(prhs[3])), t.h, "gpe_applyham");
     set_out(nlhs, plhs, 0, out);
+  } else if (cmd == "grid_transform") {
+    // the grid's fftn / ifftn / lap(phi, omega) on the device (grid2d/fft.m, ifft.m,
+    // lap.m; grid3d/fft.m, ifft.m, lap.m): out = gpe_mex('grid_transform', h, op, omega, phi)
+    // with op 0 = fft, 1 = ifft, 2 = lap
+    need(nrhs, 5, c);
+    Task& t = task_of(prhs[1]);
+    std::vector<double> tmp;
+    const double* in = complexes(prhs[4], (size_t)t.npts(), tmp, "phi");
+    mxArray* out = grid_complex(t, 1);
+    check(gpe_grid_transform(t.h, as_int(prhs[2]), mxGetScalar(prhs[3]), in, cdata(out)), t.h, "gpe_grid_transform");
+    set_out(nlhs, plhs, 0, out);
   } else if (cmd == "itp") {
     // groundstate_itp.m:20-112 after process_init_state:
     // [MU, MU2, EE, n_iter, dt_final, converged] = gpe_mex('itp', h, dt, eps, max_iter, min_iter, adjust, nargout)

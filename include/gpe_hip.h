@@ -147,6 +147,17 @@ int gpe_measure(gpe_handle* h, double out[5]);
  * Work buffers are allocated for the call ((2 ncomp + 1) npts complex values). */
 int gpe_applyham(gpe_handle* h, const double* phi, double* out, int h0_only);
 
+/* The grid's own transforms on a column-major complex array of npts values:
+ * op GPE_GRID_FFT = fftn (grid2d/fft.m:11, grid3d/fft.m:11), GPE_GRID_IFFT = ifftn with
+ * its 1/npts (grid2d/ifft.m:11, grid3d/ifft.m:11), GPE_GRID_LAP = lap(phi, omega)
+ * (grid2d/lap.m:5-9, grid3d/lap.m:5-10: ifftn(kk .* fftn(phi)), or the per-axis
+ * rotating-frame form for omega != 0).  Needs no parameters, potential or state; work
+ * buffers are allocated for the call (1 or 3 npts complex values). */
+#define GPE_GRID_FFT 0
+#define GPE_GRID_IFFT 1
+#define GPE_GRID_LAP 2
+int gpe_grid_transform(gpe_handle* h, int op, double omega, const double* in, double* out);
+
 /* groundstate_itp after process_init_state (lib/@GPEtask/groundstate_itp.m:20-112).
  * The initial state must be uploaded (gpe_set_state).  MU, MU2, EE are caller
  * arrays of max_iter entries (MU2/EE may be NULL when nargout < 3/4).  Returns
