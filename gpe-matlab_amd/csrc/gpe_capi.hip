@@ -157,6 +157,7 @@ int gpe_destroy(gpe_handle* h) {
   (void)hipFree(h->Vp);
   (void)hipFree(h->psiy);
   (void)hipFree(h->kcolB);
+  (void)hipFree(h->keB);
   (void)hipFree(h->twy);
   (void)hipFree(h->cpl);
   (void)hipFree(h->stage);
@@ -466,7 +467,7 @@ int gpe_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, double
   const double key[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // every baked-in value is fixed until gpe_split_begin
   rc = run_outer(h, h->prm.decay_rate == 0, key, [&] {
     for (int jj = 1; jj <= nblocks; ++jj) {
-      const int r = run_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec);   // 53
+      const int r = run_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec, jj > 1, jj < nblocks);   // 53
       if (r) return r;
     }
     return GPE_OK;

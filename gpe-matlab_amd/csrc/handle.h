@@ -127,8 +127,9 @@ struct gpe_handle {
   // per z pencil in the split's spectral order; v_gen counts writes of V[0]
   bool ysplit_off = false;      // GPE_YSPLIT=0 turns the two-sweep step off
   double* Vp = nullptr;
-  double2* psiy = nullptr;       // y-split step: the state in block order inside a block
+  double2* psiy = nullptr;       // y-split step: the state in block order inside an outer iteration
   double2* kcolB = nullptr;
+  double* keB = nullptr;         // y-split block ends: kinetic-sum weights in the z sweep's order
   double2* twy = nullptr;       // 256-point twiddles of the y axis
   long long v_gen = 0, vp_gen = -1;
   // one outer iteration of gpe_split_outer captured as a graph (static potential,
@@ -425,6 +426,22 @@ inline int ysplit_prepare(gpe_handle* h) {
   if (!h->kcolB) HIPCHK(h, hipMalloc(&h->kcolB, (size_t)h->nx * h->ny * sizeof(double2)));
   if (!h->Vp) HIPCHK(h, hipMalloc(&h->Vp, (size_t)h->npts * sizeof(double)));
   if (!h->psiy) HIPCHK(h, hipMalloc(&h->psiy, (size_t)h->npts * sizeof(double2)));
+  if (!h->keB) {
+    // S_YKEZ weights: (kx^2 + ky^2)/2/npts per z pencil in kcolB's order (entry k2 + 16 s
+    // of the pencils of one x is ky index bitrev4(s) + 16 k2), then kz^2/2/npts
+    auto br4 = [](int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); };
+    const double inv = 1.0 / (double)h->npts;
+    const std::vector<double>&kx = h->ax[0].k, &ky = h->ax[1].k, &kz = h->ax[2].k;
+    std::vector<double> w((size_t)h->nx * 256 + h->nz);
+    for (int ix = 0; ix < h->nx; ++ix)
+      for (int pos = 0; pos < 256; ++pos) {
+        const int iy = br4(pos >> 4) + 16 * (pos & 15);
+        w[(size_t)pos + 256 * (size_t)ix] = (kx[ix] * kx[ix] + ky[iy] * ky[iy]) / 2 * inv;
+      }
+    for (int iz = 0; iz < h->nz; ++iz) w[(size_t)h->nx * 256 + iz] = kz[iz] * kz[iz] / 2 * inv;
+    HIPCHK(h, hipMalloc(&h->keB, w.size() * sizeof(double)));
+    HIPCHK(h, hipMemcpy(h->keB, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   return ysplit_refresh_v(h);
 }
 
@@ -478,7 +495,9 @@ inline int launch_measure(gpe_handle* h, int comp, FinParams* f) {
 
 // The y-split 3D step applies: one component, ny = 256, nx and nz in {64, 128, 256},
 // nz a multiple of 8 (ysplit_eligible), with its tables and work arrays in place.
-inline bool ysplit_active(const gpe_handle* h) { return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy && h->psiy; }
+inline bool ysplit_active(const gpe_handle* h) {
+  return ysplit_eligible(h) && h->kcolB && h->Vp && h->twy && h->psiy && h->keB;
+}
 
 // x-sweep parameters of the y-split step.
 inline SweepParams ysplit_params(gpe_handle* h) {
@@ -521,26 +540,46 @@ inline int ysplit_xsweep(gpe_handle* h, int seq, const SweepParams& px, int* nb)
 
 // One block of solve_split.m:53-78 as two sweeps per step (y factor over b with the x
 // sweep, y factor over a with the z sweep): same operations as run_block, the y
-// transform factored 16 x 16.
-inline int run_block_ysplit(gpe_handle* h, double time2) {
+// transform factored 16 x 16.  blk_in: the previous block of the same outer iteration
+// left the state in psiy; blk_out: leave it there for the next one.  Without rotation
+// the hand-over is the mixed form of the steps (S_YCLOSEB, S_YKEZ, S_YOPENB); in the
+// rotating frame it is the real-space state in block order (its block-end kinetic sums
+// read the y pencils through ysplit_pos, S_YKE, and the z pencils in their permuted
+// column order, which leaves the sum over columns unchanged).
+inline int run_block_ysplit(gpe_handle* h, double time2, bool blk_in, bool blk_out) {
   int rc;
   if ((rc = ysplit_refresh_v(h))) return rc;
-  const SweepParams px = ysplit_params(h);
+  SweepParams px = ysplit_params(h);
+  px.yblk_in = blk_in ? 1 : 0;
+  px.yblk_out = blk_out ? 1 : 0;
   const SweepParams pz = ysplit_zparams(h);
   auto xs = [&](int seq, int* nb) { return ysplit_xsweep(h, seq, px, nb); };
   const bool rot = h->omega != 0.0;
-  const int s_open = rot ? gpe::S_YROPEN : gpe::S_YOPEN, s_mid = rot ? gpe::S_YRMID : gpe::S_YMID;
+  const int s_open = rot ? gpe::S_YROPEN : (blk_in ? gpe::S_YOPENB : gpe::S_YOPEN);
+  const int s_mid = rot ? gpe::S_YRMID : gpe::S_YMID;
   for (int i = 0; i < h->n_rec; ++i) {
     if ((rc = xs(i > 0 ? s_mid : s_open, nullptr))) return rc;
     if ((rc = sweep(h, 2, gpe::S_YKIN, pz))) return rc;
   }
   FinParams f = fin_params(h, gpe::FIN_BLOCK);
   int nb = 0;
-  if ((rc = xs(rot ? gpe::S_YRCLOSEM : gpe::S_YCLOSEM, &nb))) return rc;
-  f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
-  for (int a = 1; a < 3; ++a) {
-    if ((rc = sweep(h, a, gpe::S_KE, base_params(h, a, h->psi[0], h->V[0]), &nb))) return rc;
-    f.cnt[gpe::SL_KX + a] = nb;
+  if (!rot && blk_out) {
+    if ((rc = xs(gpe::S_YCLOSEB, &nb))) return rc;
+    f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = nb;
+    SweepParams pk = ysplit_zparams(h);
+    pk.kecol = h->keB;
+    pk.slot_ke = gpe::SL_KZ;
+    if ((rc = sweep(h, 2, gpe::S_YKEZ, pk, &nb))) return rc;
+    f.cnt[gpe::SL_KZ] = nb;   // the whole kinetic sum; the x and y slots stay empty
+  } else {
+    if ((rc = xs(rot ? gpe::S_YRCLOSEM : gpe::S_YCLOSEM, &nb))) return rc;
+    f.cnt[gpe::SL_A] = f.cnt[gpe::SL_B] = f.cnt[gpe::SL_C] = f.cnt[gpe::SL_KX] = nb;
+    double2* st = blk_out ? h->psiy : h->psi[0];
+    for (int a = 1; a < 3; ++a) {
+      const int seq = blk_out && a == 1 ? gpe::S_YKE : gpe::S_KE;
+      if ((rc = sweep(h, a, seq, base_params(h, a, st, h->V[0]), &nb))) return rc;
+      f.cnt[gpe::SL_KX + a] = nb;
+    }
   }
   f.renorm = (h->prm.gamma > 0 && h->prm.mu_init == 0) ? 1 : 0;
   f.NN0 = h->NN0;
@@ -550,9 +589,11 @@ inline int run_block_ysplit(gpe_handle* h, double time2) {
   return GPE_OK;
 }
 
-inline int run_block(gpe_handle* h, double time2) {
+// blk_in / blk_out: the block is not the first / last of its outer iteration (the
+// y-split step keeps the state in block order between such blocks).
+inline int run_block(gpe_handle* h, double time2, bool blk_in = false, bool blk_out = false) {
   const bool rot = h->omega != 0.0;
-  if (ysplit_active(h)) return run_block_ysplit(h, time2);
+  if (ysplit_active(h)) return run_block_ysplit(h, time2, blk_in, blk_out);
   int rc;
   double2* psi = h->psi[0];
   const double* V = h->V[0];

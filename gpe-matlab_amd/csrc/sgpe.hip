@@ -26,12 +26,14 @@ SweepParams sg_params(gpe_handle* h, int axis) {
   return p;
 }
 
-// The block of sg_block on the y-split step.
-int sg_block_ysplit(gpe_handle* h, long long step0, int* nb) {
+// The block of sg_block on the y-split step (blk_in / blk_out as for run_block).
+int sg_block_ysplit(gpe_handle* h, long long step0, int* nb, bool blk_in, bool blk_out) {
   int rc;
   if ((rc = ysplit_refresh_v(h))) return rc;
   const bool noisy = h->sg_var != 0.0;
   SweepParams px = ysplit_params(h);
+  px.yblk_in = blk_in ? 1 : 0;
+  px.yblk_out = blk_out ? 1 : 0;
   px.noise_var = h->sg_var;
   px.noise_seed = h->sg_seed;
   SweepParams pz = ysplit_zparams(h);
@@ -46,11 +48,11 @@ int sg_block_ysplit(gpe_handle* h, long long step0, int* nb) {
   return ysplit_xsweep(h, gpe::S_YCLOSEM, px, nb);   // 72-75 (only the norm is used)
 }
 
-int sg_block(gpe_handle* h, long long step0, double* ncur) {
+int sg_block(gpe_handle* h, long long step0, double* ncur, bool blk_in = false, bool blk_out = false) {
   int rc;
   if (ysplit_active(h)) {
     int nb = 0;
-    if ((rc = sg_block_ysplit(h, step0, &nb))) return rc;
+    if ((rc = sg_block_ysplit(h, step0, &nb, blk_in, blk_out))) return rc;
     FinParams f = fin_params(h, gpe::FIN_SUMS);
     f.cnt[gpe::SL_A] = nb;
     HIPCHK(h, gpe::launch_finalize(f, h->stream));
@@ -183,7 +185,7 @@ int gpe_sgpe_outer(gpe_handle* h, int j, double* time2_out, double* ncur) {
   for (int jj = 1; jj <= nblocks; ++jj) {
     time2 = time + (double)(jj - 1) * h->ddt * (double)h->n_rec;   // 53
     const long long step0 = (long long)(j - 1) * h->niter_inner + (long long)(jj - 1) * h->n_rec;
-    if ((rc = sg_block(h, step0, jj == nblocks ? ncur : nullptr))) return rc;
+    if ((rc = sg_block(h, step0, jj == nblocks ? ncur : nullptr, jj > 1, jj < nblocks))) return rc;
   }
   if (time2_out) *time2_out = time2;
   return GPE_OK;

@@ -52,7 +52,15 @@ enum Seq : int {
   S_YROPEN = 27,   // scale, block-open half phase, Kx, y factor over b
   S_YRMID = 28,    // y factor over b back, Kx, full phase, Kx, y factor over b
   S_YRCLOSEM = 29, // y factor over b back, Kx, full and closing half phase; block-end sums
-  S_NSEQ = 30
+  // y-split steps: S_KE along y of a state in block order (block ends inside an outer
+  // iteration)
+  S_YKE = 30,
+  // y-split block ends inside an outer iteration hand over in the sweeps' own mixed form
+  // (x spectral, y factor over b applied):
+  S_YCLOSEB = 31,  // S_YMID with the closing phase and the block-end sums of |psi|^2, V|psi|^2, |psi|^4
+  S_YOPENB = 32,   // S_YMID with the pending scale and the block-open half phase instead of the full phase
+  S_YKEZ = 33,     // forward z, y factor over a: the whole kinetic sum from the full spectrum
+  S_NSEQ = 34
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -125,6 +133,13 @@ struct SweepParams {
   // y-split 3D steps: the state in block order, from a block's opening x sweep (which
   // reads psi) to its closing one (which writes psi); the z sweeps run on it in place
   double2* psiy;
+  // y-split blocks of one outer iteration hand the state over in block order: the
+  // opening x sweep reads psiy instead of psi when yblk_in, the closing one writes
+  // psiy instead of psi when yblk_out
+  int yblk_in, yblk_out;
+  // S_YKEZ: (kx^2 + ky^2)/2/npts per column in the z sweep's split order (as kcol),
+  // then kz^2/2/npts per kz index
+  const double* kecol;
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

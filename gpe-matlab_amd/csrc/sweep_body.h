@@ -28,7 +28,7 @@ constexpr bool phase_seq(int s) {
 // every other sequence always runs variant 0.
 constexpr bool tunable_seq(int s) {
   return s == S_FWD || s == S_INV || s == S_KIN || s == S_KE || s == S_KEX || s == S_OPEN || s == S_MID ||
-         s == S_CLOSE || s == S_CLOSEM;
+         s == S_CLOSE || s == S_CLOSEM || s == S_YKE;
 }
 
 template <int N> constexpr bool has_variant() { return N == 256 || AltPlan<N>::has; }
@@ -49,7 +49,7 @@ template <int SEQ> struct SeqTraits {
                                  SEQ == S_RCLOSEM || SEQ == S_RITPB || SEQ == S_SOPEN || SEQ == S_SMID;
   static constexpr bool ROT_KX = SEQ == S_ROPEN || SEQ == S_RMID || SEQ == S_RCLOSEM || SEQ == S_RITPA ||
                                  SEQ == S_RITPB || SEQ == S_RKIN;
-  static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE);
+  static constexpr bool STORE = !(SEQ == S_KEX || SEQ == S_KE || SEQ == S_YKE || SEQ == S_YKEZ);
 };
 
 // Kinetic step of one pencil with a per-(row, index) table: forward, multiply,
@@ -105,7 +105,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const bool active = cgu < p.ncols;
   const unsigned cg = active ? cgu : p.ncols - 1u;
   unsigned col = (cg % p.A) * p.sa + (cg / p.A) * p.sb;
-  if constexpr (SEQ == S_YKIN || SEQ == S_YKINM) {
+  if constexpr (SEQ == S_YKIN || SEQ == S_YKINM || SEQ == S_YKEZ) {
     // z pencils of the y-split step: column y + 256 x, y = a + 16 r stored in block order
     col = (col & ~255u) + ysplit_pos(col & 15u, (col >> 4) & 15u);
   }
@@ -113,8 +113,18 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const unsigned step = (unsigned)T * p.stride;
 
   double2 v[E];
+  if constexpr (SEQ == S_YKE) {
+    // y pencils of a state in block order: value y = a + 16 r of the line is at ysplit_pos(a, r)
+    static_assert(N == 256 && !COLFAST, "256-point y pencils, contiguous");
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = psi[base + (unsigned)m * step];
+    for (int m = 0; m < E; ++m) {
+      const unsigned i = (unsigned)(t + T * m);
+      v[m] = psi[col + ysplit_pos(i & 15u, i >> 4)];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = psi[base + (unsigned)m * step];
+  }
   double vv[TR::LOAD_V ? E : 1];
   if constexpr (TR::LOAD_V) {
 #pragma unroll
@@ -126,7 +136,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const unsigned trow = p.trow ? cg % p.trow : 0u;
   const double2* __restrict__ rrow = TR::ROT_KX ? p.rtab + (size_t)trow * N : nullptr;
   const double rc =
-      (SEQ == S_KEX || SEQ == S_KE || SEQ == S_RCLOSEM) && p.rot_w != 0.0 ? p.rcoord[trow] : 0.0;
+      (SEQ == S_KEX || SEQ == S_KE || SEQ == S_YKE || SEQ == S_RCLOSEM) && p.rot_w != 0.0 ? p.rcoord[trow] : 0.0;
 
   if constexpr (SEQ == S_OPEN) {
     // solve_split.m:55-56 with the pending renormalisation (solve_split.m:72-73) folded in
@@ -202,7 +212,21 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
     DFT<16, +1>::run(v);
     row_exchange16<N, C, false>(v, lds);
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
-  } else if constexpr (SEQ == S_KEX || SEQ == S_KE) {
+  } else if constexpr (SEQ == S_YKEZ) {
+    // Block end of the y-split step (solve_split.m:74/77, the kinetic part of
+    // applyham): the closing x sweep left the state x-spectral with the y factor over b
+    // applied, so the forward z transform and the factor over a (the first half of
+    // S_YKIN) give the full spectrum; sum (kx^2 + ky^2 + kz^2)/2 |.|^2 / npts over it.
+    constexpr int G = C / 16;
+    static_assert(COLFAST && C % 16 == 0 && E == 16, "blocks of 16 pencils, 16 elements per thread");
+    fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
+    row_exchange16<N, C, true>(v, lds);
+    DFT<16, -1>::run(v);
+    const double wz = p.kecol[(size_t)p.ncols + (unsigned)(tid / G)];
+    const double* __restrict__ kc = p.kecol + tile * (unsigned)C + 16u * (unsigned)(tid % G);
+#pragma unroll
+    for (int m = 0; m < E; ++m) accK += (kc[m] + wz) * cabs2(v[m]);
+  } else if constexpr (SEQ == S_KEX || SEQ == S_KE || SEQ == S_YKE) {
     if constexpr (SEQ == S_KEX) {
       if (active) {
 #pragma unroll
@@ -356,7 +380,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
       p.partials[SL_C * p.maxwg + wg] = cc;
       p.partials[p.slot_ke * p.maxwg + wg] = k;
     }
-  } else if constexpr (SEQ == S_KE) {
+  } else if constexpr (SEQ == S_KE || SEQ == S_YKE || SEQ == S_YKEZ) {
     const double k = block_sum(accK, red);
     if (tid == 0) p.partials[p.slot_ke * p.maxwg + wg] = k;
   } else if constexpr (SEQ == S_ITPA || SEQ == S_RITPA) {
@@ -371,7 +395,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
   __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM ? 2 * N : N];
   __shared__ double red[4];
   unsigned bid = blockIdx.x, nbk = gridDim.x;
-  double2* psi = SEQ == S_YKIN || SEQ == S_YKINM ? p.psiy : p.psi;
+  constexpr bool YZ = SEQ == S_YKIN || SEQ == S_YKINM || SEQ == S_YKEZ;
+  double2* psi = YZ ? p.psiy : p.psi;
   if (p.psi2) {   // two arrays: the second half of the grid sweeps psi2 (no reductions)
     nbk >>= 1;
     if (bid >= nbk) {
@@ -379,7 +404,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
       psi = p.psi2;
     }
   }
-  const unsigned tile = SEQ == S_YKIN || SEQ == S_YKINM ? ysplit_ztile(bid, 256u / C)
+  const unsigned tile = YZ ? ysplit_ztile(bid, 256u / C)
                         : C < 16                       ? xcd_pair_tile(bid, nbk)
                                                        : bid;
   sweep_body<N, C, COLFAST, SEQ, VAR>(p, psi, tile, tile, lds, twl, red);

@@ -45,7 +45,7 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, int var, const S
       GPE_T(true, S_CLOSEM) GPE_L(true, S_ROPEN) GPE_L(true, S_RMID) GPE_L(true, S_RCLOSEM)
       GPE_L(true, S_RITPA) GPE_L(true, S_RITPB) GPE_L(true, S_SOPEN) GPE_L(true, S_SMID) GPE_L(true, S_KINM)
 #if GPE_N == 64 || GPE_N == 128 || GPE_N == 256
-      GPE_L(true, S_YKIN) GPE_L(true, S_YKINM)
+      GPE_L(true, S_YKIN) GPE_L(true, S_YKINM) GPE_L(true, S_YKEZ)
 #endif
       default: return hipErrorInvalidValue;
     }
@@ -53,6 +53,9 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, int var, const S
     switch (seq) {
       GPE_T(false, S_FWD) GPE_T(false, S_INV) GPE_T(false, S_KIN) GPE_T(false, S_KE)
       GPE_T(false, S_KEX) GPE_L(false, S_RKIN) GPE_L(false, S_KINM)
+#if GPE_N == 256
+      GPE_T(false, S_YKE)
+#endif
       default: return hipErrorInvalidValue;
     }
   }

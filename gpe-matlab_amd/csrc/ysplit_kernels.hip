@@ -1,5 +1,6 @@
 // x sweeps of the y-split 3D step for one x length (-DGPE_N=<nx>): S_YOPEN, S_YMID,
-// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65), the stochastic S_YSOPEN, S_YSMID
+// S_YCLOSEM (solve_split.m:55-56, 60-61, 60-65), their block-end forms S_YCLOSEB and
+// S_YOPENB (below), the stochastic S_YSOPEN, S_YSMID
 // (SGPEtask/solve_split.m:55-59, 69 then 59) and the rotating-frame S_YROPEN, S_YRMID,
 // S_YRCLOSEM (the same with the x half steps of ssft_kin_step.m:22-25).
 //
@@ -11,12 +12,17 @@
 // is a 16-point DFT in registers and the phase step applies.  Between the sweeps the b
 // slot r holds the spectral index k1 = bitrev4(r), times the twiddle W256^{a k1} of the
 // 16 x 16 split; the z sweep applies the factor over a.  From the opening x sweep of
-// a block to its closing one, the state lives in a second array (psiy), where the 256
-// values of every (x, z) column are in the block order ysplit_pos(a, r) (rowdft_dev.h)
-// instead of y = a + 16 r.
+// an outer iteration to the closing one of its last block, the state lives in a second
+// array (psiy), where the 256 values of every (x, z) column are in the block order
+// ysplit_pos(a, r) (rowdft_dev.h) instead of y = a + 16 r.
 //
 // The potential is read in row order (launch_ysplit_permute_v): for each b, the nx
 // values of the tile's rows are contiguous.
+//
+// Block ends inside an outer iteration: S_YCLOSEB is S_YMID with the closing phase and
+// the real-space sums, and stores the mixed form as S_YMID does; the z sweep S_YKEZ
+// then sums the kinetic term over the full spectrum, and the next block opens with
+// S_YOPENB, S_YMID with the pending scale and the block-open half phase.
 #include "philox_dev.h"
 #include "rowdft_dev.h"
 #include "sweep_body.h"
@@ -75,17 +81,23 @@ ysplit_x_kernel(const SweepParams p) {
   const unsigned ar = s8 % 16u, z = (s8 / 16u) * 8u + xcd;
   const unsigned ny = p.gny, nx = p.gnx;
   // pencil layout
-  constexpr bool NAT_IN = SEQ == S_YOPEN || SEQ == S_YSOPEN || SEQ == S_YROPEN;
-  constexpr bool NAT_OUT = SEQ == S_YCLOSEM || SEQ == S_YRCLOSEM;
+  constexpr bool OPEN_SEQ = SEQ == S_YOPEN || SEQ == S_YSOPEN || SEQ == S_YROPEN;
+  constexpr bool CLOSE_SEQ = SEQ == S_YCLOSEM || SEQ == S_YRCLOSEM;
   const int c = tid % C, t = tid / C;
   const unsigned colz = ny * (unsigned)t + ny * nx * z;
-  // y-split 3D steps: the opening sweeps read psi in natural order and the closing ones
-  // write it; everything in between is in block order in psiy.  (In place, the two
-  // orders would race: a tile's natural positions are block positions of other tiles.)
-  const double2* src = NAT_IN ? p.psi : p.psiy;
-  double2* dst = NAT_OUT ? p.psi : p.psiy;
-  const unsigned base_in = (NAT_IN ? ar + 16u * (unsigned)c : ysplit_pos(ar, (unsigned)c)) + colz;
-  const unsigned base_out = (NAT_OUT ? ar + 16u * (unsigned)c : ysplit_pos(ar, (unsigned)c)) + colz;
+  // y-split 3D steps: the opening sweep of an outer iteration reads psi in natural order
+  // and the closing sweep of its last block writes it; everything in between, block
+  // ends inside the iteration included (yblk_in / yblk_out), is in block order in psiy.
+  // (In place, the two orders would race: a tile's natural positions are block
+  // positions of other tiles.)  A natural-order tile is 16 y values 16 apart, each
+  // 16-byte piece in a line shared with 7 other tiles: the block order is the cheaper
+  // hand-over.
+  const bool nat_in = OPEN_SEQ && !p.yblk_in, nat_out = CLOSE_SEQ && !p.yblk_out;
+  const double2* src = nat_in ? p.psi : p.psiy;
+  double2* dst = nat_out ? p.psi : p.psiy;
+  const unsigned ypos = ysplit_pos(ar, (unsigned)c);
+  const unsigned base_in = (nat_in ? ar + 16u * (unsigned)c : ypos) + colz;
+  const unsigned base_out = (nat_out ? ar + 16u * (unsigned)c : ypos) + colz;
   const unsigned step = (unsigned)T * ny;
   // row layout: row q = tid is x = tid of the tile's (a, z)
   const unsigned vrow = (z * 16u + ar) * 16u * (unsigned)NT + (unsigned)tid;
@@ -202,23 +214,69 @@ ysplit_x_kernel(const SweepParams p) {
       const double aa = -(vv[b] - mu) - p.g * cabs2(x);
       u[b] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);
     }
-  } else {
+  }
+  // S_YCLOSEB: block-end sums of the real-space state (solve_split.m:65-77), accumulated
+  // as the closing phase produces it
+  double sumA = 0.0, sumB = 0.0, sumC = 0.0;
+  if constexpr (!OPEN) {
     fft_pencil<PL, +1, C>(v, t, c, lds, twl);
     row_exchange16<N, C, true>(v, lds);
     // undo the split twiddle and the y factor over b left by the previous x sweep
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) u[k1] = cmulc(v[br4c(k1)], twy[ar * (unsigned)k1]);
     DFT<16, +1>::run(u);
+    if constexpr (SEQ == S_YOPENB) {
+      // the block-open half phase on the previous block's closing state, with the
+      // pending renormalisation (solve_split.m:55-56, 72-73).  S_YCLOSEB stored the
+      // forward transforms unscaled (a z sweep would have applied the 1/npts of its
+      // kinetic table), so the inverse ones here return 16 nx times the state: the
+      // exact power of two 1/(16 nx) joins the scale.
+      const double sc = p.st->scale * (1.0 / (16.0 * (double)N));
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      const double vm = vv[b] - mu;
-      const double aa = -vm - p.g * cabs2(u[b]);                  // solve_split.m:60
-      double2 x = mul_cexp(u[b], aa * p.dr, aa * p.di, has_re);    // solve_split.m:61
-      if constexpr (SEQ == S_YCLOSEM) {
+      for (int b = 0; b < 16; ++b) {
+        const double2 x = cscale(u[b], sc);
+        const double aa = -(vv[b] - mu) - p.g * cabs2(x);
+        u[b] = mul_cexp(x, aa * p.dr * 0.5, aa * p.di * 0.5, has_re);
+      }
+    } else if constexpr (SEQ == S_YCLOSEB) {
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const double vm = vv[b] - mu;
+        const double aa = -vm - p.g * cabs2(u[b]);                  // solve_split.m:60
+        double2 x = mul_cexp(u[b], aa * p.dr, aa * p.di, has_re);    // solve_split.m:61
         const double bb = vm + p.g * cabs2(x);                       // solve_split.m:63
         x = mul_cexp(x, bb * p.dr * 0.5, bb * p.di * 0.5, has_re);   // solve_split.m:64
+        const double n = cabs2(x);
+        sumA += n;
+        sumB += vv[b] * n;
+        sumC += n * n;
+        u[b] = x;
       }
-      u[b] = x;
+    } else {
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const double vm = vv[b] - mu;
+        const double aa = -vm - p.g * cabs2(u[b]);                  // solve_split.m:60
+        double2 x = mul_cexp(u[b], aa * p.dr, aa * p.di, has_re);    // solve_split.m:61
+        if constexpr (SEQ == S_YCLOSEM) {
+          const double bb = vm + p.g * cabs2(x);                       // solve_split.m:63
+          x = mul_cexp(x, bb * p.dr * 0.5, bb * p.di * 0.5, has_re);   // solve_split.m:64
+        }
+        u[b] = x;
+      }
+    }
+  }
+  if constexpr (SEQ == S_YCLOSEB) {
+    // block end inside an outer iteration: the kinetic sum comes from the z sweep
+    // S_YKEZ over the mixed form this sweep stores, as after any other step
+    const double sa = block_sum_w<NW>(sumA, red);
+    const double sb = block_sum_w<NW>(sumB, red);
+    const double scc = block_sum_w<NW>(sumC, red);
+    if (tid == 0) {
+      const unsigned wg = blockIdx.x;
+      p.partials[SL_A * p.maxwg + wg] = sa;
+      p.partials[SL_B * p.maxwg + wg] = sb;
+      p.partials[SL_C * p.maxwg + wg] = scc;
     }
   }
   if constexpr (NOISE) {
@@ -288,6 +346,8 @@ hipError_t GPE_CAT(launch_ysplit_x_, GPE_N)(int seq, const SweepParams& p, hipSt
     case S_YROPEN: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YROPEN>), grid, block, 0, s, p); break;
     case S_YRMID: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YRMID>), grid, block, 0, s, p); break;
     case S_YRCLOSEM: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YRCLOSEM>), grid, block, 0, s, p); break;
+    case S_YCLOSEB: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YCLOSEB>), grid, block, 0, s, p); break;
+    case S_YOPENB: hipLaunchKernelGGL((ysplit_x_kernel<N, S_YOPENB>), grid, block, 0, s, p); break;
     default: return hipErrorInvalidValue;
   }
   if (nblocks) *nblocks = (int)grid.x;

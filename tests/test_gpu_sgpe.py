@@ -110,3 +110,18 @@ def test_sgpe_two_sweep_step_with_more_x_tiles_than_resident():
     print(f"SGPE two-sweep 64x256x256: vs four-sweep {e_4:.2e}", flush=True)
     assert e_4 < 1e-12
     assert np.allclose(tg.history["n"], t4.history["n"], rtol=1e-11, atol=0)
+
+
+def test_sgpe_two_sweep_blocks_hand_over_in_block_order():
+    """30 inner steps of n_recalc = 10: the block ends inside an outer iteration keep the
+    state in block order; same noise stream as the oracle and the four-sweep step."""
+    n, xmax = (64, 256, 64), (6, 8, 5)
+    tg, to = _with_schedule("1", lambda: _pair(3, n, xmax, ecut=6.0))
+    pg = _with_schedule("1", lambda: tg.solve_split(0.005, 30, 2))
+    t4, _ = _with_schedule("0", lambda: _pair(3, n, xmax, ecut=6.0))
+    p4 = _with_schedule("0", lambda: t4.solve_split(0.005, 30, 2))
+    po = to.solve_split(0.005, 30, 2)
+    e_o, e_4 = rel_l2(pg, po), rel_l2(pg, p4)
+    print(f"SGPE two-sweep 3 blocks per outer: vs oracle {e_o:.2e}, vs four-sweep {e_4:.2e}", flush=True)
+    assert e_o < 1e-11 and e_4 < 1e-12
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)

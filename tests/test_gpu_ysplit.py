@@ -230,3 +230,62 @@ def test_ysplit_rotating_frame_with_more_x_tiles_than_resident():
     print(f"y-split rotating 64x256x256: vs four sweeps {e_4:.3e}", flush=True)
     assert e_4 < TOL
     assert np.allclose(tg.history["mu"], t4.history["mu"], rtol=TOL, atol=0)
+
+
+@pytest.mark.parametrize("omega", [0.0, 0.4])
+@pytest.mark.parametrize("gamma", [0.0, 0.02])
+def test_ysplit_blocks_of_an_outer_iteration_hand_over_in_block_order(omega, gamma):
+    """30 inner steps of n_recalc = 10: three blocks per outer iteration.  The first block
+    opens from the natural order and the last closes to it; the block ends between them
+    stay in block order.  Damped runs renormalise and update mu at every block end;
+    against the oracle and the four-sweep step."""
+    import gpe_hip
+    n, xmax = (64, 256, 64), (6, 9, 5)
+    G, O = _pair(n, xmax)
+    phi = _phi(G, 23)
+    props = dict(gamma=gamma)
+    tg = _make(G, gpe_hip, phi, **props)
+    to = _make(O, oracle, phi, **props)
+    t4 = _four_sweep(lambda: _make(G, gpe_hip, phi, **props))
+    for t in (tg, to, t4):
+        t.omega = omega
+    pg = tg.solve_split(0.002, 30, 2)
+    po = to.solve_split(0.002, 30, 2)
+    p4 = _four_sweep(lambda: t4.solve_split(0.002, 30, 2))
+    e_o, e_4 = rel_l2(pg, po), rel_l2(pg, p4)
+    print(f"y-split 3 blocks per outer, omega={omega} gamma={gamma}: vs oracle {e_o:.3e}, "
+          f"vs four sweeps {e_4:.3e}", flush=True)
+    assert e_o < TOL and e_4 < TOL
+    assert np.allclose(tg.history["mu"], to.history["mu"], rtol=TOL, atol=0)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=TOL, atol=0)
+    dev = tg._device()
+    dev.profile_enable(True)
+    tg.solve_split(0.002, 30, 3)
+    prof = dev.profile_read()
+    dev.profile_enable(False)
+    # block ends: the two inside the iteration hand over in block order (without rotation
+    # in the steps' mixed form, whose kinetic sum is one z sweep, S_YKEZ; in the rotating
+    # frame in real space, y kinetic sums through S_YKE); the y kinetic sums in natural
+    # order are the last block end's and the initial mu of the resumed run (solve_split.m:41)
+    if omega == 0.0:
+        assert prof[(31, 0)][1] == 2 and prof[(32, 0)][1] == 2 and prof[(33, 2)][1] == 2, prof
+    else:
+        assert prof[(30, 1)][1] == 2, prof
+    assert prof[(7, 1)][1] == 2, prof
+
+
+def test_ysplit_blocks_hand_over_with_more_x_tiles_than_resident():
+    """The block-order hand-over at 64 x 256 x 256 (4096 x-sweep work-groups): three blocks
+    per outer iteration equal the four-sweep step."""
+    import gpe_hip
+    n, xmax = (64, 256, 256), (6, 9, 9)
+    G, _ = _pair(n, xmax)
+    phi = _phi(G, 29)
+    tg = _make(G, gpe_hip, phi, gamma=0.02)
+    t4 = _four_sweep(lambda: _make(G, gpe_hip, phi, gamma=0.02))
+    pg = tg.solve_split(0.002, 30, 2)
+    p4 = _four_sweep(lambda: t4.solve_split(0.002, 30, 2))
+    e_4 = rel_l2(pg, p4)
+    print(f"y-split 3 blocks per outer 64x256x256: vs four sweeps {e_4:.3e}", flush=True)
+    assert e_4 < TOL
+    assert np.allclose(tg.history["mu"], t4.history["mu"], rtol=TOL, atol=0)
