@@ -167,6 +167,8 @@ int gpe_destroy(gpe_handle* h) {
     (void)hipFree(h->kcol_sp[i]);
     (void)hipFree(h->kax_sp[i]);
   }
+  (void)hipFree(h->kax_sp2);
+  for (double* t : h->keax_nx) (void)hipFree(t);
   (void)hipFree(h->nop);
   (void)hipFree(h->psi_old);
   (void)hipFree(h->adj_V0);

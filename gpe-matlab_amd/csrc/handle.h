@@ -57,6 +57,11 @@ struct gpe_handle {
   double2* kax = nullptr;       // kinetic factor per mid-axis index (incl. 1/npts)
   double2* kcol_sp[2] = {nullptr, nullptr};  // spinor: exp(-kk dt/2) and exp(-kk dt) exp(kk dt/2)
   double2* kax_sp[2] = {nullptr, nullptr};
+  // spinor block ends inside an outer iteration (3D): kax_sp[0] times nx (a block
+  // opening from the stored x spectrum F_x psi / nx) and the y and z k^2/2/n tables
+  // times nx (kinetic sums of that spectrum)
+  double2* kax_sp2 = nullptr;
+  double* keax_nx[3] = {nullptr, nullptr, nullptr};
   double cosom[2] = {1, 0}, sinomm[2] = {0, 0}, sinomp[2] = {0, 0};  // scalar Rabi coefficients
   double* nop = nullptr;        // ITP stored operator
   double2* psi_old = nullptr;   // ITP convergence reference
@@ -284,7 +289,7 @@ inline int mid_axis(const gpe_handle* h) { return h->ndims == 3 ? 2 : 1; }
 // Kinetic tables for complex dt: kcol[col] * kax[i] = exp(-kk dt) / npts
 // (set_kinop.m:9) split into the mid-spectrum axis and its columns.
 inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, double2* kax, double frac = 1.0,
-                         double frac2 = 0.0, double2* kcolB = nullptr) {
+                         double frac2 = 0.0, double2* kcolB = nullptr, double axmul = 1.0) {
   const int ma = mid_axis(h);
   const Axis& m = h->ax[ma];
   std::vector<double2> col, axv(m.n);
@@ -306,7 +311,7 @@ inline int build_kinetic(gpe_handle* h, std::complex<double> dt, double2* kcol, 
       col[ix] = make_double2(e.real(), e.imag());
     }
   }
-  const double inv = 1.0 / (double)h->npts;
+  const double inv = axmul / (double)h->npts;
   for (int i = 0; i < m.n; ++i) {
     const double kk = m.k[i] * m.k[i] / 2;
     std::complex<double> e = std::exp(-kk * dt * frac);

@@ -13,10 +13,13 @@ void rabi_scalars(gpe_handle* h);
 int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax);
 // One block of n_rec steps.  tape (if set) receives the real-space state of both
 // components before each step (2 * n_rec arrays); reduce = false skips the
-// block-end chemical potential and renormalisation.
-int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce);
-// SP_KEX + y/z kinetic sums of both components into the partials; fills f's counts.
-int spinor_reduce(gpe_handle* h, gpe::FinParams* f, bool scaled);
+// block-end chemical potential and renormalisation.  blk_in / blk_out (3D with
+// reduce): the block opens from / ends in the stored x spectrum of a neighbouring
+// block of the same outer iteration instead of the real-space state.
+int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool blk_in = false, bool blk_out = false);
+// SP_KEX (SP_KEXS when spectral) + y/z kinetic sums of both components into the
+// partials; fills f's counts.
+int spinor_reduce(gpe_handle* h, gpe::FinParams* f, bool scaled, bool spectral = false);
 }  // namespace gpe_spinor
 
 namespace gpe {

@@ -7,6 +7,11 @@
 //   close : exp(+kk dt/2) folded into the last step's kinetic    (97-99)
 //           multiply (table exp(-kk dt) exp(kk dt/2)), inverse x
 //   reduce: ncur, optional renormalisation, mu = <phi|H phi>/NNN (100-115)
+// Between the sweeps of a step the state is the x spectrum S = F_x psi / nx.  In 3D,
+// block ends inside an outer iteration keep it: the sums are taken from S (SP_KEXS and
+// kinetic sums with nx-scaled weights), and the next block opens from S with its
+// kinetic half step (the pending renormalisation in the y transform, nx in the kz
+// table) instead of an inverse and a forward x sweep.
 // 3D sweep order per step: x (both components, SP_PR), then for each component
 // y forward, z forward.kinetic.inverse, y inverse.  2D: x (SP_PR), y kinetic.
 #include "handle.h"
@@ -62,7 +67,7 @@ int spinor_sweep(gpe_handle* h, int seq, const gpe::SpinorParams& p, int* nb) {
   if (h->prof && e0 && e1) {
     hipEventRecord(e1, h->stream);
     // spinor sweeps are reported in the (seq, axis) slots of S_MID / S_KEX / S_CLOSE on x
-    const int slot = seq == gpe::SP_PR ? gpe::S_MID : seq == gpe::SP_KEX ? gpe::S_KEX : gpe::S_CLOSE;
+    const int slot = seq == gpe::SP_PR ? gpe::S_MID : seq == gpe::SP_KEX || seq == gpe::SP_KEXS ? gpe::S_KEX : gpe::S_CLOSE;
     h->prof_recs.push_back({slot * 3 + 0, {e0, e1}});
   }
   if (e != hipSuccess) return fail(h, GPE_EHIP, "spinor sweep launch: %s", hipGetErrorString(e));
@@ -85,8 +90,10 @@ void rabi_scalars(gpe_handle* h) {
 }
 
 // Reductions for ncur and mu on the real-space state of both components
-// (scaled: each component multiplied by st->sc[k] in the x-sweep).
-int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled) {
+// (scaled: each component multiplied by st->sc[k] in the x-sweep), or, when
+// `spectral`, on the stored x spectrum S = F_x psi / nx of a block end inside an outer
+// iteration (3D; y and z kinetic sums of S weighted by nx k^2/2/n, Parseval along x).
+int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled, bool spectral) {
   int rc, nb = 0;
   gpe::SpinorParams p = spinor_params(h);
   p.apply_sc = scaled ? 1 : 0;
@@ -94,12 +101,13 @@ int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled) {
     p.V1 = h->Vmu[0];
     p.V2 = h->Vmu[1];
   }
-  if ((rc = spinor_sweep(h, gpe::SP_KEX, p, &nb))) return rc;
+  if ((rc = spinor_sweep(h, spectral ? gpe::SP_KEXS : gpe::SP_KEX, p, &nb))) return rc;
   for (int s : {gpe::SL_A, gpe::SL_B, gpe::SL_C, gpe::SL_C12, gpe::SL_C2, gpe::SL_X, gpe::SL_KX, gpe::SL_KX2})
     f->cnt[s] = nb;
   for (int comp = 0; comp < 2; ++comp)
     for (int a = 1; a < h->ndims; ++a) {
       SweepParams q = base_params(h, a, h->psi[comp], h->V[comp]);
+      if (spectral) q.keax = h->keax_nx[a];
       q.slot_ke = (comp == 0 ? gpe::SL_KX : gpe::SL_KX2) + a;
       if ((rc = sweep(h, a, gpe::S_KE, q, &nb))) return rc;
       f->cnt[q.slot_ke] = nb;
@@ -154,17 +162,34 @@ static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) 
   return GPE_OK;
 }
 
-int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
+// blk_in / blk_out (3D, reduce): the previous / next block of the same outer iteration
+// hands the state over as the stored x spectrum (see the top of this file).
+int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool blk_in, bool blk_out) {
   int rc;
-  // open: x forward of both components (with the pending renormalisation), then
-  // exp(-kk dt/2)
-  {
+  if (!blk_in) {
+    // open: x forward of both components (with the pending renormalisation), then
+    // exp(-kk dt/2)
     SweepParams p = base_params(h, 0, h->psi[0], nullptr);
     p.psi2 = h->psi[1];
     p.apply_scale = 1;
     if ((rc = sweep(h, 0, gpe::S_FWD, p))) return rc;
+    if ((rc = spinor_rest_of_step_both(h, h->kcol_sp[0], h->kax_sp[0]))) return rc;
+  } else {
+    // open from S = F_x psi / nx: the y transform applies the pending renormalisation
+    // and the kz table carries the nx an x forward transform would have supplied
+    SweepParams f = base_params(h, 1, h->psi[0], nullptr);
+    f.psi2 = h->psi[1];
+    f.apply_scale = 1;
+    if ((rc = sweep(h, 1, gpe::S_FWD, f))) return rc;
+    SweepParams k = base_params(h, 2, h->psi[0], nullptr);
+    k.psi2 = h->psi[1];
+    k.kcol = h->kcol_sp[0];
+    k.kax = h->kax_sp2;
+    if ((rc = sweep(h, 2, gpe::S_KIN, k))) return rc;
+    SweepParams v = base_params(h, 1, h->psi[0], nullptr);
+    v.psi2 = h->psi[1];
+    if ((rc = sweep(h, 1, gpe::S_INV, v))) return rc;
   }
-  if ((rc = spinor_rest_of_step_both(h, h->kcol_sp[0], h->kax_sp[0]))) return rc;
   for (int i = 0; i < h->n_rec; ++i) {
     gpe::SpinorParams p = spinor_params(h);
     if (tape) {
@@ -175,14 +200,14 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce) {
     const bool last = i + 1 == h->n_rec;
     if ((rc = spinor_rest_of_step_both(h, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax))) return rc;
   }
-  {
+  if (!blk_out) {
     SweepParams p = base_params(h, 0, h->psi[0], nullptr);
     p.psi2 = h->psi[1];
     if ((rc = sweep(h, 0, gpe::S_INV, p))) return rc;
   }
   if (!reduce) return GPE_OK;
   FinParams f = fin_params(h, gpe::FIN_SPINOR);
-  if ((rc = spinor_reduce(h, &f, false))) return rc;
+  if ((rc = spinor_reduce(h, &f, false, blk_out))) return rc;
   f.renorm = h->prm.gamma > 0 ? 1 : 0;
   f.NN0 = h->NN0;
   f.tau = h->prm.decay_rate;
@@ -235,6 +260,20 @@ int spinor_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_o
   if ((rc = build_kinetic(h, dt, h->kcol, h->kax))) return rc;                 // ekk  (43)
   if ((rc = build_kinetic(h, dt, h->kcol_sp[0], h->kax_sp[0], 0.5))) return rc;   // ekkp (44)
   if ((rc = build_kinetic(h, dt, h->kcol_sp[1], h->kax_sp[1], 1.0, -0.5))) return rc;  // ekk * ekkm (43, 45)
+  if (h->ndims == 3) {
+    // tables of the block ends inside an outer iteration (spinor_block): kax_sp[0] x nx
+    // (kcol_sp[0] is rebuilt unchanged) and the y and z k^2/2/n tables x nx
+    if (!h->kax_sp2) HIPCHK(h, hipMalloc(&h->kax_sp2, h->ax[2].n * sizeof(double2)));
+    if ((rc = build_kinetic(h, dt, h->kcol_sp[0], h->kax_sp2, 0.5, 0.0, nullptr, (double)h->nx))) return rc;
+    for (int a = 1; a < 3; ++a) {
+      if (h->keax_nx[a]) continue;
+      const Axis& ax = h->ax[a];
+      std::vector<double> t(ax.n);
+      for (int i = 0; i < ax.n; ++i) t[i] = ax.k[i] * ax.k[i] / 2 / (double)ax.n * (double)h->nx;
+      HIPCHK(h, hipMalloc(&h->keax_nx[a], t.size() * sizeof(double)));
+      HIPCHK(h, hipMemcpy(h->keax_nx[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
   double m[5];
   if ((rc = spinor_measure(h, m))) return rc;
   // 52-57: mu = sum_j real(inner(phi_j, H phi_j)) / NN0
@@ -255,7 +294,7 @@ int spinor_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_o
 int spinor_split_block(gpe_handle* h, double time2, double* muc, double* ncur) {
   int rc;
   rabi_scalars(h);
-  if ((rc = spinor_block(h, time2, nullptr, true))) return rc;
+  if ((rc = spinor_block(h, time2, nullptr, true, false, false))) return rc;
   DevState s;
   if ((rc = read_state(h, &s))) return rc;
   if ((rc = check_finite(h, s, "gpe_split_block"))) return rc;
@@ -277,7 +316,9 @@ int spinor_split_outer(gpe_handle* h, int j, double* time2_out, double* muc, dou
                          (double)reinterpret_cast<uintptr_t>(h->cpl), 0.0, 0.0};
   rc = run_outer(h, h->prm.decay_rate == 0, key, [&] {
     for (int jj = 1; jj <= nblocks; ++jj) {
-      const int r = spinor_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec, nullptr, true);   // 69
+      const bool d3 = h->ndims == 3;
+      const int r = spinor_block(h, time + (double)(jj - 1) * h->ddt * (double)h->n_rec, nullptr, true,
+                                 d3 && jj > 1, d3 && jj < nblocks);   // 69
       if (r) return r;
     }
     return GPE_OK;

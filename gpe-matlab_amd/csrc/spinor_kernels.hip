@@ -12,6 +12,10 @@
 // SP_KEX: reductions of the block-end chemical potential on the real-space
 //         state (solve_split.m:100-115, GPEtaskSpinor.m:55-92) plus the x part of
 //         the kinetic energy of both components.
+// SP_KEXS: the same sums from the stored x spectrum S = F_x psi / nx that the steps
+//         leave between the sweeps (block ends inside an outer iteration): the x
+//         kinetic part straight from S, then the inverse x-FFT for the real-space
+//         sums; nothing is stored.
 // SP_ITPA/B/C: the x-sweeps of one two-component imaginary-time iteration
 //         (GPEtaskSpinor/groundstate_itp.m:49-88, GPEtask2comp/groundstate_itp.m:58-90):
 //         A renormalises the real-space state, stores exp(-(V_k + sum_j g_kj n_j) dt)
@@ -350,6 +354,17 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
   } else {
     double accA = 0.0, accB = 0.0, accC1 = 0.0, accC12 = 0.0, accC2 = 0.0, accX = 0.0, k1 = 0.0, k2 = 0.0;
     const double own1 = k ? 0.0 : 1.0;   // sums that belong to component 1's lane only
+    if constexpr (SEQ == SP_KEXS) {
+      // k_i^2/2/n |F_x psi|^2 = k_i^2/2/n n^2 |S|^2; S holds index k_i = t + T m
+      constexpr double NN = (double)N * (double)N;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double e = (p.keax[t + T * m] * NN) * cabs2(v[m]);
+        k1 += own1 * e;
+        k2 += (1.0 - own1) * e;
+      }
+      fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);   // the real-space state
+    }
     if (p.apply_sc) {
       const double sk = p.st->sc[k];
 #pragma unroll
@@ -369,12 +384,14 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       const double2 ow = cmul(om, o);
       accX += own1 * (v[m].x * ow.x + v[m].y * ow.y);   // Re conj(psi1) Omega psi2
     }
-    fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
+    if constexpr (SEQ == SP_KEX) {
+      fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const double e = p.keax[t + T * m] * cabs2(v[m]);
-      k1 += own1 * e;
-      k2 += (1.0 - own1) * e;
+      for (int m = 0; m < E; ++m) {
+        const double e = p.keax[t + T * m] * cabs2(v[m]);
+        k1 += own1 * e;
+        k2 += (1.0 - own1) * e;
+      }
     }
     if (!active) accA = accB = accC1 = accC12 = accC2 = accX = k1 = k2 = 0.0;
     const double vals[8] = {accA, accB, accC1, accC12, accC2, accX, k1, k2};
@@ -427,6 +444,8 @@ hipError_t GPE_CAT(launch_spinor_sweep_, GPE_N)(int seq, const SpinorParams& p, 
       hipLaunchKernelGGL((spinor_kernel<N, C2, SP_PR, CK_SCALAR>), g, b, 0, s, p);
   } else if (seq == SP_KEX) {
     hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEX, CK_SCALAR>), g, b, 0, s, p);
+  } else if (seq == SP_KEXS) {
+    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEXS, CK_SCALAR>), g, b, 0, s, p);
   } else if (seq == SP_ADJ) {
     hipLaunchKernelGGL((spinor_adj_kernel<N, C2>), g, b, 0, s, p);
   } else if (seq == SP_ITPA) {

@@ -143,7 +143,7 @@ struct SweepParams {
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).
-enum SpSeq : int { SP_PR = 0, SP_KEX = 1, SP_ADJ = 2, SP_ITPA = 3, SP_ITPB = 4, SP_ITPC = 5 };
+enum SpSeq : int { SP_PR = 0, SP_KEX = 1, SP_ADJ = 2, SP_ITPA = 3, SP_ITPB = 4, SP_ITPC = 5, SP_KEXS = 6 };
 
 struct SpinorParams {
   double2* psi1;

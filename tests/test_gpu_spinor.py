@@ -217,3 +217,25 @@ def test_spinor_array_coupling_mutated_in_place_is_uploaded():
         res[name] = t.solve_split(0.004, 20, 4)
     err = max(rel_l2(res["gpu"][0], res["oracle"][0]), rel_l2(res["gpu"][1], res["oracle"][1]))
     assert err < TOL
+
+
+@pytest.mark.parametrize("kind", ["array", "damped", "scalar"])
+def test_spinor_blocks_of_an_outer_iteration_hand_over_the_x_spectrum(kind):
+    """3D, 30 inner steps of n_recalc = 10: the block ends inside an outer iteration reduce
+    the stored x spectrum and the next block opens from it (no real-space round trip);
+    the first and last block of the iteration use the real-space state.  Against the
+    oracle, with the mu / n histories."""
+    g = [[0.5, 0.45], [0.4, 0.6]]
+    dims, n, xmax = 3, (48, 32, 32), (6, 5, 5)
+    if kind == "scalar":
+        tg, to = _spinor_pair(dims, n, xmax, 0.5 * np.exp(0.3j), g, gamma=0.01)
+    else:
+        tg, to = _spinor_pair(dims, n, xmax, lambda G: 0.5 * np.exp(2j * G.mesh.x), g,
+                              gamma=0.02 if kind == "damped" else 0.0)
+    pg = tg.solve_split(0.002, 30, 2)
+    po = to.solve_split(0.002, 30, 2)
+    err = max(rel_l2(pg[0], po[0]), rel_l2(pg[1], po[1]))
+    print(f"spinor 3 blocks per outer {kind}: rel-L2 = {err:.3e}", flush=True)
+    assert err < TOL
+    assert np.allclose(tg.history["mu"], np.real(to.history["mu"]), rtol=1e-11, atol=1e-13)
+    assert np.allclose(tg.history["n"], to.history["n"], rtol=1e-11, atol=0)
