@@ -74,15 +74,16 @@ def plan_steps(steps, warmup, n_recalc=10):
     its first outer iteration captures the outer-iteration graph, and after the host
     work before the timed leg (state upload, kinetic tables) the per-step time of
     the 256^3 step settles over about 80 steps (10-step outer iterations measured at
-    4.3 ms, falling to 3.86 ms).  Outer iterations are 50 steps when both rounded
-    counts allow it, else n_recalc steps.  Returns (ni, warm_outer, timed_outer);
+    4.3 ms, falling to 3.86 ms).  An outer iteration is the largest whole number of
+    blocks up to 50 steps (5 blocks) that divides both rounded counts: solve_split(dt,
+    ni, ...) with ni in {50, 40, 30, 20, 10}.  Returns (ni, warm_outer, timed_outer);
     the steps actually timed are ni * timed_outer >= steps.
     """
     steps = max(1, int(steps))
     warmup = max(0, int(warmup))
     s = -(-steps // n_recalc) * n_recalc
     w = max(MIN_WARMUP, -(-warmup // n_recalc) * n_recalc)
-    ni = 50 if (s % 50 == 0 and w % 50 == 0) else n_recalc
+    ni = next(b * n_recalc for b in range(5, 0, -1) if s % (b * n_recalc) == 0 and w % (b * n_recalc) == 0)
     return ni, w // ni, s // ni
 
 

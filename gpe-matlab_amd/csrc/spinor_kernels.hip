@@ -217,9 +217,10 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       v[m] = mul_cexp(r, -a * p.dr, -a * p.di, has_re);
     }
     fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
+    const unsigned bs = opaque(base);   // the store offset is formed here, not kept live
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psik[bs + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ADJ) {
     // v holds this lane's adjoint component (x in k space); ust1/ust2 hold the
@@ -300,9 +301,10 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       if (active) pk[idx] = exp_cw(-a * p.dr);                 // exp(-tmp2 dt) (70-71)
     }
     fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
+    const unsigned bs = opaque(base);   // the store offset is formed here, not kept live
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psik[bs + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ITPB) {
     fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
@@ -331,18 +333,20 @@ __device__ __forceinline__ void spinor_body(const SpinorParams& p) {
       }
     }
     fft_pencil<PL, -1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
+    const unsigned bs = opaque(base);   // the store offset is formed here, not kept live
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psik[bs + (unsigned)m * step] = v[m];
     }
   } else if constexpr (SEQ == SP_ITPC) {
     fft_pencil<PL, +1, C2, COLFAST>(v, t, c, ldsk, twl, swz);
     double acc = 0.0;
 #pragma unroll
     for (int m = 0; m < E; ++m) acc += cabs2(v[m]);
+    const unsigned bs = opaque(base);   // the store offset is formed here, not kept live
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) psik[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psik[bs + (unsigned)m * step] = v[m];
     }
     if (!active) acc = 0.0;
     const double r1 = block_sum(k ? 0.0 : acc, red);
@@ -409,6 +413,14 @@ __global__ void __launch_bounds__(256) spinor_kernel(const SpinorParams p) {
   spinor_body<N, C2, SEQ, CK>(p);
 }
 
+// SP_KEXS unconstrained takes more than 256 registers (one wave per SIMD): its potential
+// and coupling loads are hoisted above the inverse transform.  Two waves per SIMD.
+template <int N, int C2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+spinor_kexs_kernel(const SpinorParams p) {
+  spinor_body<N, C2, SP_KEXS, CK_SCALAR>(p);
+}
+
 // The transposed step has the largest live state (about 276 registers when the
 // compiler is unconstrained: one wave per SIMD).  Two waves per SIMD with a few
 // spilled registers run it 1.4x faster at 192^3 (255 vs 349 us per sweep);
@@ -445,7 +457,7 @@ hipError_t GPE_CAT(launch_spinor_sweep_, GPE_N)(int seq, const SpinorParams& p, 
   } else if (seq == SP_KEX) {
     hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEX, CK_SCALAR>), g, b, 0, s, p);
   } else if (seq == SP_KEXS) {
-    hipLaunchKernelGGL((spinor_kernel<N, C2, SP_KEXS, CK_SCALAR>), g, b, 0, s, p);
+    hipLaunchKernelGGL((spinor_kexs_kernel<N, C2>), g, b, 0, s, p);
   } else if (seq == SP_ADJ) {
     hipLaunchKernelGGL((spinor_adj_kernel<N, C2>), g, b, 0, s, p);
   } else if (seq == SP_ITPA) {

@@ -344,9 +344,11 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   }
 
   if constexpr (TR::STORE) {
+    // the store offset is formed here rather than kept live from the loads
+    const unsigned bs = opaque(base);
     if (active) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) psi[base + (unsigned)m * step] = v[m];
+      for (int m = 0; m < E; ++m) psi[bs + (unsigned)m * step] = v[m];
     }
   }
 

@@ -16,12 +16,13 @@ def _bench():
 
 
 @pytest.mark.parametrize("steps,warmup,expect", [
-    ((20, 5), None, (10, 10, 2)),     # 20 timed steps after the minimum warm-up of 100
+    ((20, 5), None, (20, 5, 1)),      # 20 timed steps after the minimum warm-up of 100
     ((7, 3), None, (10, 10, 1)),
     ((500, 50), None, (50, 2, 10)),
     ((1, 0), None, (10, 10, 1)),
     ((100, 50), None, (50, 2, 2)),
     ((30, 200), None, (10, 20, 3)),
+    ((40, 0), None, (20, 5, 2)),
 ])
 def test_plan_steps(steps, warmup, expect):
     b = _bench()
@@ -37,7 +38,8 @@ def test_plan_steps_never_exits():
     for s in range(1, 120):
         for w in range(0, 60):
             ni, wo, to = b.plan_steps(s, w)
-            assert ni in (10, 50) and to >= 1 and wo >= 1
+            assert ni in (10, 20, 30, 40, 50) and to >= 1 and wo >= 1
+            assert ni * to >= s and ni * to - s < 10 and ni * wo >= b.MIN_WARMUP
 
 
 def test_gpus_above_one_exits_with_message():
