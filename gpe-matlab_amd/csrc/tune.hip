@@ -26,11 +26,14 @@ using namespace gpe_detail;
 namespace {
 
 // Relative gain a non-default plan combination needs to be recorded.
-constexpr double kMargin = 0.03;
+constexpr double kMargin = 0.015;
+// Each timing covers at least this long (repetitions from a pilot timing): four steps of
+// a 256^2 grid are 60 us, too short to resolve the few percent between plans.
+constexpr double kMinTimedUs = 2000.0;
 
 // Average time in microseconds of one steady-state step under the handle's current
 // variants, over `reps` steps after two untimed ones.
-int time_step(gpe_handle* h, double* us) {
+int time_step(gpe_handle* h, int reps, double* us) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(h, hipEventCreate(&e0));
   hipError_t e = hipEventCreate(&e1);
@@ -38,7 +41,7 @@ int time_step(gpe_handle* h, double* us) {
     (void)hipEventDestroy(e0);
     return fail(h, GPE_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
   }
-  const int warm = 2, reps = 4;
+  const int warm = 2;
   auto sw = [&](int axis, int seq) {
     if (e != hipSuccess) return;
     SweepParams p = base_params(h, axis, h->psi[0], h->V[0]);
@@ -105,6 +108,13 @@ int gpe_tune(gpe_handle* h, int remeasure, double* times_us) {
     if (e != hipSuccess) rc = fail(h, GPE_EHIP, "tuning: %s", hipGetErrorString(e));
     const int keep[3] = {h->var[0], h->var[1], h->var[2]};
     const int ncomb = 1 << axes.size();
+    int reps = 4;
+    if (rc == GPE_OK) {
+      for (int a = 0; a < 3; ++a) h->var[a] = base[a];
+      double pilot = 0;
+      rc = time_step(h, 4, &pilot);
+      if (rc == GPE_OK && pilot > 0) reps = (int)std::fmin(256.0, std::fmax(4.0, std::ceil(kMinTimedUs / pilot)));
+    }
     std::vector<double> best_t(ncomb, 1e300);
     // rounds run every combination in turn, so clock ramps and other traffic fall on
     // all of them; each combination keeps its best round
@@ -113,7 +123,7 @@ int gpe_tune(gpe_handle* h, int remeasure, double* times_us) {
         for (int a = 0; a < 3; ++a) h->var[a] = base[a];
         for (size_t i = 0; i < axes.size(); ++i) h->var[axes[i]] = (m >> i) & 1;
         double us = 0;
-        rc = time_step(h, &us);
+        rc = time_step(h, reps, &us);
         if (us < best_t[m]) best_t[m] = us;
       }
     for (int a = 0; a < 3; ++a) h->var[a] = keep[a];
@@ -123,8 +133,8 @@ int gpe_tune(gpe_handle* h, int remeasure, double* times_us) {
     (void)hipFree(save);
     if (rc) return rc;
     // variant 0 is the default; another combination must beat the one that keeps
-    // every measured axis at variant 0 by kMargin to replace it (step timings of a few
-    // steps differ by about a percent between rounds)
+    // every measured axis at variant 0 by kMargin to replace it (timings of 2 ms differ
+    // by a few tenths of a percent between rounds)
     int mbest = 0;
     for (int m = 1; m < ncomb; ++m)
       if (best_t[m] < best_t[mbest]) mbest = m;
