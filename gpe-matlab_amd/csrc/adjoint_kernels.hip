@@ -93,11 +93,12 @@ __global__ void __launch_bounds__(256) adj_x_kernel(const AdjParams p) {
     fft_pencil<PL, -1, C, COLFAST>(u, t, c, lds, twl);
     fft_pencil<PL, -1, C, COLFAST>(l, t, c, lds, twl);
   }
+  const unsigned bs = opaque(base);   // the store offset is formed here, not kept live
   if (active) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      p.psi[base + (unsigned)m * step] = u[m];
-      p.lam[base + (unsigned)m * step] = l[m];
+      p.psi[bs + (unsigned)m * step] = u[m];
+      p.lam[bs + (unsigned)m * step] = l[m];
     }
   }
 }
