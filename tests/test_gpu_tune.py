@@ -168,6 +168,6 @@ def test_ensemble_tune_records_every_pair():
     assert set(ens.tune_times) == {(0, 128), (0, 256), (1, 64), (1, 128), (1, 256)}
     for (a, n), (t0, t1) in ens.tune_times.items():
         assert t0 > 0 and t1 > 0
-        assert _lib.wisdom_get(2, a, n) == (1 if t1 < 0.97 * t0 else 0)
+        assert _lib.wisdom_get(2, a, n) == (1 if t1 < 0.985 * t0 else 0)   # the tuner's 1.5 % margin
     assert _lib.wisdom_get(2, 0, 192) == -1
     ens.solve_split(0.004, 10, 1)
