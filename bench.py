@@ -41,7 +41,7 @@ SEQ_BYTES = {"fwd": 32, "inv": 32, "kin": 32, "open": 40, "mid": 40, "close": 40
              "itpa": 40, "itpb": 48, "closem": 40, "rkin": 32, "ropen": 40, "rmid": 40, "rclosem": 40,
              "ritpa": 40, "ritpb": 48, "sopen": 40, "smid": 40, "kinm": 32,
              "ykin": 32, "yopen": 40, "ymid": 40, "yclosem": 40, "ysopen": 40, "ysmid": 40, "ykinm": 32,
-             "yropen": 40, "yrmid": 40, "yrclosem": 40, "yke": 16, "ycloseb": 40, "yopenb": 40, "ykez": 16}
+             "yropen": 40, "yrmid": 40, "yrclosem": 40, "yke": 16, "ycloseb": 40, "yopenb": 40, "ykez": 16, "kine": 32}
 STEP_BYTES_3D = 136  # 4 sweeps x 32 B + 8 B of V per step
 STEP_BYTES_3D_YSPLIT = 72  # two-sweep step: 2 x 32 B + 8 B of V
 

@@ -168,6 +168,8 @@ int gpe_destroy(gpe_handle* h) {
     (void)hipFree(h->kax_sp[i]);
   }
   (void)hipFree(h->kax_sp2);
+  (void)hipFree(h->kecolS);
+  (void)hipFree(h->kezS);
   for (double* t : h->keax_nx) (void)hipFree(t);
   (void)hipFree(h->nop);
   (void)hipFree(h->psi_old);

@@ -62,6 +62,10 @@ struct gpe_handle {
   // times nx (kinetic sums of that spectrum)
   double2* kax_sp2 = nullptr;
   double* keax_nx[3] = {nullptr, nullptr, nullptr};
+  // 3D spinor block ends: kinetic-sum weights of the last kinetic sweep (S_KINE):
+  // (kx^2 + ky^2)/2 npts per z pencil (kcol's order) and kz^2/2 npts per kz index
+  double* kecolS = nullptr;
+  double* kezS = nullptr;
   double cosom[2] = {1, 0}, sinomm[2] = {0, 0}, sinomp[2] = {0, 0};  // scalar Rabi coefficients
   double* nop = nullptr;        // ITP stored operator
   double2* psi_old = nullptr;   // ITP convergence reference

@@ -18,8 +18,9 @@ int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax
 // block of the same outer iteration instead of the real-space state.
 int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool blk_in = false, bool blk_out = false);
 // SP_KEX (SP_KEXS when spectral) + y/z kinetic sums of both components into the
-// partials; fills f's counts.
-int spinor_reduce(gpe_handle* h, gpe::FinParams* f, bool scaled, bool spectral = false);
+// partials; fills f's counts.  ke_groups > 0: the kinetic sums are already in SL_KZ /
+// SL_KZ2 over that many work-groups per component (S_KINE), and no kinetic sweeps run.
+int spinor_reduce(gpe_handle* h, gpe::FinParams* f, bool scaled, bool spectral = false, int ke_groups = 0);
 }  // namespace gpe_spinor
 
 namespace gpe {

@@ -7,6 +7,9 @@
 //   close : exp(+kk dt/2) folded into the last step's kinetic    (97-99)
 //           multiply (table exp(-kk dt) exp(kk dt/2)), inverse x
 //   reduce: ncur, optional renormalisation, mu = <phi|H phi>/NNN (100-115)
+// In 3D the last kinetic sweep of a block (S_KINE) also sums kk |.|^2 over the spectrum
+// it has just multiplied, which is the block-end state's: the kinetic energy needs no
+// sweeps of its own.
 // Between the sweeps of a step the state is the x spectrum S = F_x psi / nx.  In 3D,
 // block ends inside an outer iteration keep it: the sums are taken from S (SP_KEXS and
 // kinetic sums with nx-scaled weights), and the next block opens from S with its
@@ -93,7 +96,7 @@ void rabi_scalars(gpe_handle* h) {
 // (scaled: each component multiplied by st->sc[k] in the x-sweep), or, when
 // `spectral`, on the stored x spectrum S = F_x psi / nx of a block end inside an outer
 // iteration (3D; y and z kinetic sums of S weighted by nx k^2/2/n, Parseval along x).
-int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled, bool spectral) {
+int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled, bool spectral, int ke_groups) {
   int rc, nb = 0;
   gpe::SpinorParams p = spinor_params(h);
   p.apply_sc = scaled ? 1 : 0;
@@ -104,7 +107,13 @@ int spinor_reduce(gpe_handle* h, FinParams* f, bool scaled, bool spectral) {
   if ((rc = spinor_sweep(h, spectral ? gpe::SP_KEXS : gpe::SP_KEX, p, &nb))) return rc;
   for (int s : {gpe::SL_A, gpe::SL_B, gpe::SL_C, gpe::SL_C12, gpe::SL_C2, gpe::SL_X, gpe::SL_KX, gpe::SL_KX2})
     f->cnt[s] = nb;
-  for (int comp = 0; comp < 2; ++comp)
+  if (ke_groups > 0) {
+    // the whole kinetic sum of each component is in SL_KZ / SL_KZ2 (S_KINE); the x sums
+    // of the sweep above are left out
+    f->cnt[gpe::SL_KX] = f->cnt[gpe::SL_KX2] = 0;
+    f->cnt[gpe::SL_KZ] = f->cnt[gpe::SL_KZ2] = ke_groups;
+  }
+  for (int comp = 0; comp < 2 && ke_groups <= 0; ++comp)
     for (int a = 1; a < h->ndims; ++a) {
       SweepParams q = base_params(h, a, h->psi[comp], h->V[comp]);
       if (spectral) q.keax = h->keax_nx[a];
@@ -141,7 +150,9 @@ int spinor_rest_of_step(gpe_handle* h, double2* psi, double2* kcol, double2* kax
 
 // spinor_rest_of_step for both components, one launch per sweep: the second
 // component runs on the second half of each grid (same operations per component).
-static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) {
+// ke_groups (3D): the kinetic sweep also sums each component's kinetic energy of the
+// result into SL_KZ / SL_KZ2; *ke_groups receives its work-groups per component.
+static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax, int* ke_groups = nullptr) {
   int rc;
   const int ka = h->ndims == 3 ? 2 : 1;
   if (h->ndims == 3) {
@@ -153,7 +164,16 @@ static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) 
   k.psi2 = h->psi[1];
   k.kcol = kcol;
   k.kax = kax;
-  if ((rc = sweep(h, ka, gpe::S_KIN, k))) return rc;
+  const bool ke = ke_groups && h->ndims == 3;
+  if (ke) {
+    k.kecol = h->kecolS;
+    k.keax = h->kezS;
+    k.slot_ke = gpe::SL_KZ;
+    k.slot_ke2 = gpe::SL_KZ2;
+  }
+  int nb = 0;
+  if ((rc = sweep(h, ka, ke ? gpe::S_KINE : gpe::S_KIN, k, &nb))) return rc;
+  if (ke) *ke_groups = nb / 2;
   if (h->ndims == 3) {
     SweepParams v = base_params(h, 1, h->psi[0], nullptr);
     v.psi2 = h->psi[1];
@@ -165,7 +185,7 @@ static int spinor_rest_of_step_both(gpe_handle* h, double2* kcol, double2* kax) 
 // blk_in / blk_out (3D, reduce): the previous / next block of the same outer iteration
 // hands the state over as the stored x spectrum (see the top of this file).
 int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool blk_in, bool blk_out) {
-  int rc;
+  int rc, ke_groups = 0;
   if (!blk_in) {
     // open: x forward of both components (with the pending renormalisation), then
     // exp(-kk dt/2)
@@ -198,7 +218,9 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool b
     }
     if ((rc = spinor_sweep(h, gpe::SP_PR, p, nullptr))) return rc;
     const bool last = i + 1 == h->n_rec;
-    if ((rc = spinor_rest_of_step_both(h, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax))) return rc;
+    if ((rc = spinor_rest_of_step_both(h, last ? h->kcol_sp[1] : h->kcol, last ? h->kax_sp[1] : h->kax,
+                                       last && reduce && h->ndims == 3 ? &ke_groups : nullptr)))
+      return rc;
   }
   if (!blk_out) {
     SweepParams p = base_params(h, 0, h->psi[0], nullptr);
@@ -207,7 +229,7 @@ int spinor_block(gpe_handle* h, double time2, double2* tape, bool reduce, bool b
   }
   if (!reduce) return GPE_OK;
   FinParams f = fin_params(h, gpe::FIN_SPINOR);
-  if ((rc = spinor_reduce(h, &f, false, blk_out))) return rc;
+  if ((rc = spinor_reduce(h, &f, false, blk_out, ke_groups))) return rc;
   f.renorm = h->prm.gamma > 0 ? 1 : 0;
   f.NN0 = h->NN0;
   f.tau = h->prm.decay_rate;
@@ -265,6 +287,19 @@ int spinor_split_begin(gpe_handle* h, double ddt, int niter_inner, double* muc_o
     // (kcol_sp[0] is rebuilt unchanged) and the y and z k^2/2/n tables x nx
     if (!h->kax_sp2) HIPCHK(h, hipMalloc(&h->kax_sp2, h->ax[2].n * sizeof(double2)));
     if ((rc = build_kinetic(h, dt, h->kcol_sp[0], h->kax_sp2, 0.5, 0.0, nullptr, (double)h->nx))) return rc;
+    if (!h->kecolS) {
+      const std::vector<double>&kx = h->ax[0].k, &ky = h->ax[1].k, &kz = h->ax[2].k;
+      const double np = (double)h->npts;
+      std::vector<double> wc((size_t)h->nx * h->ny), wz(h->nz);
+      for (int ix = 0; ix < h->nx; ++ix)
+        for (int iy = 0; iy < h->ny; ++iy)
+          wc[(size_t)iy + (size_t)h->ny * ix] = (kx[ix] * kx[ix] + ky[iy] * ky[iy]) / 2 * np;
+      for (int iz = 0; iz < h->nz; ++iz) wz[iz] = kz[iz] * kz[iz] / 2 * np;
+      HIPCHK(h, hipMalloc(&h->kecolS, wc.size() * sizeof(double)));
+      HIPCHK(h, hipMemcpy(h->kecolS, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(h, hipMalloc(&h->kezS, wz.size() * sizeof(double)));
+      HIPCHK(h, hipMemcpy(h->kezS, wz.data(), wz.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     for (int a = 1; a < 3; ++a) {
       if (h->keax_nx[a]) continue;
       const Axis& ax = h->ax[a];

@@ -60,7 +60,10 @@ enum Seq : int {
   S_YCLOSEB = 31,  // S_YMID with the closing phase and the block-end sums of |psi|^2, V|psi|^2, |psi|^4
   S_YOPENB = 32,   // S_YMID with the pending scale and the block-open half phase instead of the full phase
   S_YKEZ = 33,     // forward z, y factor over a: the whole kinetic sum from the full spectrum
-  S_NSEQ = 34
+  // S_KIN that also sums kk |.|^2 over the spectrum it multiplied (3D spinor blocks end
+  // with a kinetic step, so that spectrum is the block-end state's)
+  S_KINE = 34,
+  S_NSEQ = 35
 };
 
 // Reduction slots of the partial-sum buffer.  Spinor runs use C for |psi1|^4,
@@ -138,8 +141,10 @@ struct SweepParams {
   // psiy instead of psi when yblk_out
   int yblk_in, yblk_out;
   // S_YKEZ: (kx^2 + ky^2)/2/npts per column in the z sweep's split order (as kcol),
-  // then kz^2/2/npts per kz index
+  // then kz^2/2/npts per kz index.  S_KINE: its column weights (keax: those of the
+  // swept axis)
   const double* kecol;
+  int slot_ke2;              // S_KINE with psi2: slot of the second array's sums
 };
 
 // Two-component x-sweep parameters (GPEtaskSpinor).

@@ -28,7 +28,7 @@ constexpr bool phase_seq(int s) {
 // every other sequence always runs variant 0.
 constexpr bool tunable_seq(int s) {
   return s == S_FWD || s == S_INV || s == S_KIN || s == S_KE || s == S_KEX || s == S_OPEN || s == S_MID ||
-         s == S_CLOSE || s == S_CLOSEM || s == S_YKE;
+         s == S_CLOSE || s == S_CLOSEM || s == S_YKE || s == S_KINE;
 }
 
 template <int N> constexpr bool has_variant() { return N == 256 || AltPlan<N>::has; }
@@ -96,7 +96,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
   const int c = COLFAST ? tid % C : tid / T;
   const int t = COLFAST ? tid / C : tid % T;
   for (int i = tid; i < N; i += 256) twl[i] = p.tw[i];
-  if constexpr (SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM) {
+  if constexpr (SEQ == S_KIN || SEQ == S_KINE || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM) {
     for (int i = tid; i < N; i += 256) twl[N + i] = p.kax[i];
     __syncthreads();  // single-pass plans have no exchange barrier before the first use
   }
@@ -172,12 +172,19 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_INV) {
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
-  } else if constexpr (SEQ == S_KIN) {
+  } else if constexpr (SEQ == S_KIN || SEQ == S_KINE) {
     // the per-index factors share the twiddle buffer's second half (staged above)
     const double2 kc = p.kcol[cg];
     fft_pencil<PL, -1, C, COLFAST>(v, t, c, lds, twl);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(cmul(kc, twl[N + t + T * m]), v[m]);
+    if constexpr (SEQ == S_KINE) {
+      if (active) {
+        const double wc = p.kecol[cg];
+#pragma unroll
+        for (int m = 0; m < E; ++m) accK += (wc + p.keax[t + T * m]) * cabs2(v[m]);
+      }
+    }
     fft_pencil<PL, +1, C, COLFAST>(v, t, c, lds, twl);
   } else if constexpr (SEQ == S_YKIN || SEQ == S_YKINM) {
     // Every 16 consecutive pencils of the tile are one y block (a = 0..15 of
@@ -382,9 +389,10 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
       p.partials[SL_C * p.maxwg + wg] = cc;
       p.partials[p.slot_ke * p.maxwg + wg] = k;
     }
-  } else if constexpr (SEQ == S_KE || SEQ == S_YKE || SEQ == S_YKEZ) {
+  } else if constexpr (SEQ == S_KE || SEQ == S_YKE || SEQ == S_YKEZ || SEQ == S_KINE) {
     const double k = block_sum(accK, red);
-    if (tid == 0) p.partials[p.slot_ke * p.maxwg + wg] = k;
+    const int slot = SEQ == S_KINE && p.psi2 && psi == p.psi2 ? p.slot_ke2 : p.slot_ke;
+    if (tid == 0) p.partials[slot * p.maxwg + wg] = k;
   } else if constexpr (SEQ == S_ITPA || SEQ == S_RITPA) {
     const double a = block_sum(accA, red);
     if (tid == 0) p.partials[SL_A * p.maxwg + wg] = a;
@@ -394,7 +402,7 @@ __device__ __forceinline__ void sweep_body(const SweepParams& p, double2* __rest
 template <int N, int C, bool COLFAST, int SEQ, int VAR = 0>
 __global__ void __launch_bounds__(256) sweep_kernel(const SweepParams p) {
   __shared__ double lds[Lds<N, C, COLFAST>::SIZE];
-  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM ? 2 * N : N];
+  __shared__ double2 twl[SEQ == S_KIN || SEQ == S_KINE || SEQ == S_KINM || SEQ == S_YKIN || SEQ == S_YKINM ? 2 * N : N];
   __shared__ double red[4];
   unsigned bid = blockIdx.x, nbk = gridDim.x;
   constexpr bool YZ = SEQ == S_YKIN || SEQ == S_YKINM || SEQ == S_YKEZ;

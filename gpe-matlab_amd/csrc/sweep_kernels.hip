@@ -19,7 +19,7 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, int var, const S
   // pencils per work-group follow the sequence's plan (sweep_cols); variant 1 exists
   // for the tunable sequences of lengths with a second plan
   if (!has_variant<N>() || !tunable_seq(seq)) var = 0;
-  if (p.psi2 && seq != S_FWD && seq != S_INV && seq != S_KIN) return hipErrorInvalidValue;
+  if (p.psi2 && seq != S_FWD && seq != S_INV && seq != S_KIN && seq != S_KINE) return hipErrorInvalidValue;
 #define GPE_LV(CF, S, V)                                                                               \
   {                                                                                                    \
     constexpr int C = sweep_cols<N, S, V>();                                                           \
@@ -42,7 +42,7 @@ hipError_t GPE_CAT(launch_sweep_, GPE_N)(int seq, bool colfast, int var, const S
     switch (seq) {
       GPE_T(true, S_FWD) GPE_T(true, S_INV) GPE_T(true, S_KIN) GPE_T(true, S_OPEN) GPE_T(true, S_MID)
       GPE_T(true, S_CLOSE) GPE_T(true, S_KEX) GPE_T(true, S_KE) GPE_L(true, S_ITPA) GPE_L(true, S_ITPB)
-      GPE_T(true, S_CLOSEM) GPE_L(true, S_ROPEN) GPE_L(true, S_RMID) GPE_L(true, S_RCLOSEM)
+      GPE_T(true, S_CLOSEM) GPE_T(true, S_KINE) GPE_L(true, S_ROPEN) GPE_L(true, S_RMID) GPE_L(true, S_RCLOSEM)
       GPE_L(true, S_RITPA) GPE_L(true, S_RITPB) GPE_L(true, S_SOPEN) GPE_L(true, S_SMID) GPE_L(true, S_KINM)
 #if GPE_N == 64 || GPE_N == 128 || GPE_N == 256
       GPE_L(true, S_YKIN) GPE_L(true, S_YKINM) GPE_L(true, S_YKEZ)

@@ -473,7 +473,7 @@ SEQ_NAMES = {0: "fwd", 1: "inv", 2: "kin", 3: "open", 4: "mid", 5: "close", 6: "
              10: "closem", 11: "rkin", 12: "ropen", 13: "rmid", 14: "rclosem", 15: "ritpa", 16: "ritpb",
              17: "sopen", 18: "smid", 19: "kinm", 20: "ykin", 21: "yopen", 22: "ymid", 23: "yclosem",
              24: "ysopen", 25: "ysmid", 26: "ykinm", 27: "yropen", 28: "yrmid", 29: "yrclosem",
-             30: "yke", 31: "ycloseb", 32: "yopenb", 33: "ykez"}
+             30: "yke", 31: "ycloseb", 32: "yopenb", 33: "ykez", 34: "kine"}
 AXIS_NAMES = {0: "x", 1: "y", 2: "z"}
 
 
